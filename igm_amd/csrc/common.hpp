@@ -1,0 +1,85 @@
+// Shared helpers for the gfx950 kernels: error reporting for the C-ABI and
+// cross-lane exchange inside a 64-wide wavefront.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace igm {
+
+// Last error message of the calling host thread (read by igm_last_error()).
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define IGM_CHECK_ARG(cond, msg)                         \
+  do {                                                   \
+    if (!(cond)) {                                       \
+      ::igm::set_error(std::string(__func__) + ": " + (msg)); \
+      return -1;                                         \
+    }                                                    \
+  } while (0)
+
+#define IGM_CHECK_LAUNCH()                                                    \
+  do {                                                                        \
+    hipError_t e_ = hipGetLastError();                                        \
+    if (e_ != hipSuccess) {                                                   \
+      ::igm::set_error(std::string(__func__) + ": launch failed: " + hipGetErrorString(e_)); \
+      return -2;                                                              \
+    }                                                                         \
+  } while (0)
+
+constexpr int kWave = 64;
+
+// Lane l receives the value held by lane l+1 (lane 63 receives 0).
+// DPP wave_shl:1 -- one VALU move, no LDS traffic.
+__device__ __forceinline__ float lane_from_right(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
+}
+// Lane l receives the value held by lane l-1 (lane 0 receives 0).
+// DPP wave_shr:1.
+__device__ __forceinline__ float lane_from_left(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double lane_from_right(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xF, 0xF, true);
+  hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_from_left(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, true);
+  hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wavefront (every lane gets the total).
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, kWave);
+  return x;
+}
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, kWave);
+  return x;
+}
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) x = fmaxf(x, __shfl_xor(x, off, kWave));
+  return x;
+}
+
+// x^e for x > 0 via the hardware log2/exp2 (fp32) or libm (fp64).
+__device__ __forceinline__ float pow_pos(float x, float e) {
+  return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
+}
+__device__ __forceinline__ double pow_pos(double x, double e) { return pow(x, e); }
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+
+}  // namespace igm

@@ -1,0 +1,719 @@
+// Blatter-Pattyn energy of IGM's ice-flow solver on gfx950: one fused pass
+// that evaluates the staggered-grid energy, its adjoint (dJ/dU, dJ/dV) and,
+// optionally, the legacy-Adam update, with a float64 energy reduction.
+//
+// Reference: igm/processes/iceflow/energy/cost_shear.py:103-179,
+// cost_gravity.py:204-249, cost_sliding_weertman.py:272-292, energy.py:12-22,
+// utils/gradient/compute_gradient.py:3-16, solve/solve.py:166-205.
+//
+// Work decomposition ("column marching"): a wavefront owns a strip of 64
+// node columns and a block of `rows` node rows.  Lane l holds node column
+// ic = 62*strip - 1 + l; lanes 1..62 own their column, lanes 0 and 63 are a
+// one-node halo.  The wave walks up the rows: at step jc it holds node rows
+// jc and jc+1 of U and V (all Nz levels) in registers, computes the cell
+// (jc, ic) -- the cell whose lower-left node is (jc, ic) -- taking the right
+// neighbour's nodes from lane l+1 with a DPP move, and scatters the cell's
+// adjoint to its 4 corner nodes: the two left corners stay in the lane, the
+// two right ones move to lane l+1 with a DPP move.  Node row jc is complete
+// once cell rows jc-1 and jc are done; the upper-corner contributions of cell
+// row jc are carried in registers to the next step.  No LDS, no atomics: the
+// gradient is a deterministic gather.
+//
+// Per-cell coefficients that do not depend on (U, V) are computed once per
+// solve by energy_coeffs_kernel (7 planes, 28 B per cell).
+#include "common.hpp"
+#include "../../include/igm_amd.h"
+
+#include <cmath>
+#include <mutex>
+
+namespace igm {
+
+namespace {
+thread_local std::string g_last_error;
+}
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+constexpr int kPlanes = 7;     // q, w, gx, gy, cs, bx, by
+constexpr int kStripOut = 62;  // owned columns per wave
+constexpr int kPassBlock = 256;
+constexpr int kWavesPerBlock = kPassBlock / kWave;
+constexpr int kDefaultRows = 8;
+
+// --------------------------------------------------------- coefficients --
+//
+// Plane meaning for the cell with lower-left node (j, i) (S = number of cells):
+//   0  q  = 0.25 * damp / max(H, thr)      damp = 1, or 0.01 where stag4h(C) <= 0
+//   1  w  = H * Bs / (p * S)               Bs = stag4h(2 A^(-1/n)), p = 1 + 1/n
+//   2  gx = 0.25 * rho*g*1e-6 * H * sx / S (sx, sy: staggered slope of usurf)
+//   3  gy = 0.25 * rho*g*1e-6 * H * sy / S
+//   4  cs = stag4h(C) / (s * S)            s = 1 + 1/m
+//   5  bx, 6 by: staggered slope of the bed usurf - thk
+// The 0.25 factors absorb the 1/4 of the horizontal node average, which the
+// pass keeps unscaled (sum of the 4 corner nodes).
+template <typename T>
+__global__ void __launch_bounds__(256)
+energy_coeffs_kernel(const T* __restrict__ thk, const T* __restrict__ usurf,
+                     const T* __restrict__ arrh, const T* __restrict__ slid,
+                     const T* __restrict__ dX, int64_t fb, int64_t fr, int64_t fc,
+                     const int32_t* __restrict__ extents, int Ny, int Nx, T inv_n, T inv_p,
+                     T inv_s, T thr, T rho_g, T* __restrict__ coef, T* __restrict__ hscale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  const int b = blockIdx.z;
+  if (i >= Nx) return;
+  const int ny = extents ? min(extents[2 * b], Ny) : Ny;
+  const int nx = extents ? min(extents[2 * b + 1], Nx) : Nx;
+  const int64_t plane = (int64_t)Ny * Nx;
+  T* cb = coef + (int64_t)b * kPlanes * plane + (int64_t)j * Nx + i;
+  const int64_t base = (int64_t)b * fb;
+  if (i == 0 && j == 0) hscale[b] = T(1) / (T(4) * dX[base]);
+  if (!(j < ny - 1 && i < nx - 1)) {
+#pragma unroll
+    for (int p = 0; p < kPlanes; ++p) cb[p * plane] = T(0);
+    return;
+  }
+  const int64_t o00 = base + (int64_t)j * fr + (int64_t)i * fc;
+  const int64_t o01 = o00 + fc;       // (j, i+1)
+  const int64_t o10 = o00 + fr;       // (j+1, i)
+  const int64_t o11 = o00 + fr + fc;  // (j+1, i+1)
+  auto st4 = [](T a11, T a10, T a01, T a00) { return (a11 + a10 + a01 + a00) / T(4); };
+  const T t00 = thk[o00], t01 = thk[o01], t10 = thk[o10], t11 = thk[o11];
+  const T s00 = usurf[o00], s01 = usurf[o01], s10 = usurf[o10], s11 = usurf[o11];
+  const T d00 = dX[o00], d01 = dX[o01], d10 = dX[o10], d11 = dX[o11];
+  const T H = st4(t11, t10, t01, t00);
+  const T Bs = st4(T(2) * pow(arrh[o11], -inv_n), T(2) * pow(arrh[o10], -inv_n),
+                   T(2) * pow(arrh[o01], -inv_n), T(2) * pow(arrh[o00], -inv_n));
+  const T Cs = st4(slid[o11], slid[o10], slid[o01], slid[o00]);
+  // compute_gradient.py:10-14 on s = usurf (surface) and s = usurf - thk (bed)
+  auto gx = [&](T a00, T a01, T a10, T a11) {
+    const T e0 = T(2) * (a01 - a00) / (d01 + d00);
+    const T e1 = T(2) * (a11 - a10) / (d11 + d10);
+    return T(0.5) * (e1 + e0);
+  };
+  auto gy = [&](T a00, T a01, T a10, T a11) {
+    const T e0 = T(2) * (a10 - a00) / (d10 + d00);
+    const T e1 = T(2) * (a11 - a01) / (d11 + d01);
+    return T(0.5) * (e1 + e0);
+  };
+  const T sx = gx(s00, s01, s10, s11), sy = gy(s00, s01, s10, s11);
+  const T b00 = s00 - t00, b01 = s01 - t01, b10 = s10 - t10, b11 = s11 - t11;
+  const T bx = gx(b00, b01, b10, b11), by = gy(b00, b01, b10, b11);
+  const T inv_S = T(1) / (T(ny - 1) * T(nx - 1));
+  const T damp = Cs > T(0) ? T(1) : T(0.01);
+  const T Hc = H > thr ? H : thr;
+  cb[0 * plane] = T(0.25) * damp / Hc;
+  cb[1 * plane] = H * Bs * inv_p * inv_S;
+  cb[2 * plane] = T(0.25) * rho_g * H * sx * inv_S;
+  cb[3 * plane] = T(0.25) * rho_g * H * sy * inv_S;
+  cb[4 * plane] = Cs * inv_s * inv_S;
+  cb[5 * plane] = bx;
+  cb[6 * plane] = by;
+}
+
+// ----------------------------------------------------------------- pass --
+
+template <typename T>
+struct PassArgs {
+  const T* U;
+  const T* V;
+  T* gU;
+  T* gV;
+  T* Uo;
+  T* Vo;
+  T* mU;
+  T* vU;
+  T* mV;
+  T* vV;
+  const T* coef;
+  const T* hscale;
+  double* partials;
+  const int32_t* stop;
+  int64_t bs, ks, rs, cs;
+  int64_t plane;
+  int Ny, Nx, rows, nstrips, nrowblk, blocks_per_batch;
+  T dz[IGM_MAX_NZ];   // layer thickness dzeta_l
+  T idz[IGM_MAX_NZ];  // 1/dzeta_l
+  T wz[IGM_MAX_NZ];   // level weight of the gravity term: (dzeta_{k-1} + dzeta_k)/2
+  T e_glen;           // (p-2)/2
+  T min_sr2, max_sr2, regu2;
+  T e_slid;           // s/2 - 1
+  T s_slid;           // s
+  T regu_w2;
+  T alpha, omb1, omb2, eps;  // Adam: alpha, 1-beta1, 1-beta2, epsilon
+};
+
+template <typename T, int NZ, int MODE>
+__global__ void __launch_bounds__(kPassBlock)
+energy_pass_kernel(const PassArgs<T> a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int b = blockIdx.y;
+  double e_sh = 0.0, e_sl = 0.0, e_gr = 0.0;
+  const bool stopped = (a.stop != nullptr) && (*a.stop != 0);
+
+  if (!stopped && wid < a.nstrips * a.nrowblk) {
+    const int strip = wid % a.nstrips;
+    const int rb = wid / a.nstrips;
+    const int ic = strip * kStripOut - 1 + lane;
+    const bool col_ok = (ic >= 0) && (ic < a.Nx);
+    const bool own = (lane >= 1) && (lane <= kStripOut) && col_ok;
+    const int j0 = rb * a.rows;
+    const int j1 = min(j0 + a.rows, a.Ny);
+    const int64_t boff = (int64_t)b * a.bs + (int64_t)ic * a.cs;
+    const T* __restrict__ Ug = a.U + boff;
+    const T* __restrict__ Vg = a.V + boff;
+    const T* __restrict__ cb = a.coef + (int64_t)b * kPlanes * a.plane + ic;
+    const T hh = a.hscale[b];
+
+    auto load_row = [&](T (&u)[NZ], T (&v)[NZ], int row) {
+      if (col_ok && row >= 0 && row < a.Ny) {
+        const int64_t o = (int64_t)row * a.rs;
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          u[k] = Ug[o + k * a.ks];
+          v[k] = Vg[o + k * a.ks];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          u[k] = T(0);
+          v[k] = T(0);
+        }
+      }
+    };
+
+    T up[NZ], vp[NZ], uc[NZ], vc[NZ], cu[NZ], cv[NZ];
+    load_row(up, vp, j0 - 1);
+    load_row(uc, vc, j0);
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      cu[k] = T(0);
+      cv[k] = T(0);
+    }
+
+    for (int jc = j0 - 1; jc < j1; ++jc) {
+      T un[NZ], vn[NZ];
+      load_row(un, vn, jc + 2);  // prefetch for the next step
+
+      T q = 0, w = 0, gx = 0, gy = 0, csl = 0, bx = 0, by = 0;
+      if (col_ok && jc >= 0) {
+        const T* c = cb + (int64_t)jc * a.Nx;
+        q = c[0];
+        w = c[a.plane];
+        gx = c[2 * a.plane];
+        gy = c[3 * a.plane];
+        csl = c[4 * a.plane];
+        bx = c[5 * a.plane];
+        by = c[6 * a.plane];
+      }
+      const bool emit_row = own && (jc >= j0);
+      const int64_t onode = (int64_t)jc * a.rs;  // node (jc, ic), level 0
+      T mu[NZ], vu_[NZ], mv[NZ], vv_[NZ];
+      if constexpr (MODE == 2) {
+        if (emit_row) {
+          T* mUg = a.mU + (int64_t)b * a.bs + (int64_t)ic * a.cs + onode;
+          T* vUg = a.vU + (int64_t)b * a.bs + (int64_t)ic * a.cs + onode;
+          T* mVg = a.mV + (int64_t)b * a.bs + (int64_t)ic * a.cs + onode;
+          T* vVg = a.vV + (int64_t)b * a.bs + (int64_t)ic * a.cs + onode;
+#pragma unroll
+          for (int k = 0; k < NZ; ++k) {
+            mu[k] = mUg[k * a.ks];
+            vu_[k] = vUg[k * a.ks];
+            mv[k] = mVg[k * a.ks];
+            vv_[k] = vVg[k * a.ks];
+          }
+        }
+      }
+
+      // Scatter of the adjoint of level k to the 4 corner nodes, then output
+      // of node (jc, ic), level k.
+      auto emit = [&](int k, T gux, T guy, T gsu, T gvx, T gvy, T gsv) {
+        {
+          const T qx = gux * hh, qy = guy * hh;
+          const T A = gsu - qx, B = gsu + qx;
+          const T c00 = A - qy, c10 = A + qy, c01 = B - qy, c11 = B + qy;
+          const T g = cu[k] + c00 + lane_from_left(c01);
+          cu[k] = c10 + lane_from_left(c11);
+          if (emit_row) {
+            if constexpr (MODE == 1) {
+              a.gU[(int64_t)b * a.bs + (int64_t)ic * a.cs + onode + k * a.ks] = g;
+            } else if constexpr (MODE == 2) {
+              T m = mu[k], v = vu_[k];
+              m += (g - m) * a.omb1;
+              v += (g * g - v) * a.omb2;
+              const int64_t o = (int64_t)b * a.bs + (int64_t)ic * a.cs + onode + k * a.ks;
+              a.Uo[o] = up[k] - (m * a.alpha) / (sqrt_(v) + a.eps);
+              a.mU[o] = m;
+              a.vU[o] = v;
+            }
+          }
+        }
+        {
+          const T qx = gvx * hh, qy = gvy * hh;
+          const T A = gsv - qx, B = gsv + qx;
+          const T c00 = A - qy, c10 = A + qy, c01 = B - qy, c11 = B + qy;
+          const T g = cv[k] + c00 + lane_from_left(c01);
+          cv[k] = c10 + lane_from_left(c11);
+          if (emit_row) {
+            if constexpr (MODE == 1) {
+              a.gV[(int64_t)b * a.bs + (int64_t)ic * a.cs + onode + k * a.ks] = g;
+            } else if constexpr (MODE == 2) {
+              T m = mv[k], v = vv_[k];
+              m += (g - m) * a.omb1;
+              v += (g * g - v) * a.omb2;
+              const int64_t o = (int64_t)b * a.bs + (int64_t)ic * a.cs + onode + k * a.ks;
+              a.Vo[o] = vp[k] - (m * a.alpha) / (sqrt_(v) + a.eps);
+              a.mV[o] = m;
+              a.vV[o] = v;
+            }
+          }
+        }
+      };
+
+      // level sweep; layer l = k-1 is handled once level k is known
+      T p_ux = 0, p_uy = 0, p_vx = 0, p_vy = 0, p_su = 0, p_sv = 0;  // level k-1
+      T g_ux = 0, g_uy = 0, g_vx = 0, g_vy = 0, g_su = 0, g_sv = 0;  // its adjoint
+      T c_sh = 0, c_gr = 0, c_sl = 0;
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) {
+        const T su = up[k] + uc[k], du = uc[k] - up[k];
+        const T sv = vp[k] + vc[k], dv = vc[k] - vp[k];
+        const T su_r = lane_from_right(su), du_r = lane_from_right(du);
+        const T sv_r = lane_from_right(sv), dv_r = lane_from_right(dv);
+        const T ux = (su_r - su) * hh, uy = (du + du_r) * hh;  // half of the level derivative
+        const T vx = (sv_r - sv) * hh, vy = (dv + dv_r) * hh;
+        const T sU = su + su_r, sV = sv + sv_r;                 // 4 x node average
+        T n_ux = 0, n_uy = 0, n_vx = 0, n_vy = 0;
+        T n_su = a.wz[k] * gx, n_sv = a.wz[k] * gy;             // gravity (linear)
+        c_gr += a.wz[k] * (gx * sU + gy * sV);
+        if (k == 0) {  // Weertman sliding on the basal level
+          const T ub = T(0.25) * sU, vb = T(0.25) * sV;
+          const T c2 = ub * bx + vb * by;
+          const T N = (ub * ub + vb * vb) + a.regu_w2 + c2 * c2;
+          const T t = pow_pos(N, a.e_slid);  // N^(s/2 - 1)
+          c_sl = csl * N * t;
+          const T gs = T(0.25) * csl * a.s_slid * t;
+          n_su += gs * (ub + c2 * bx);
+          n_sv += gs * (vb + c2 * by);
+        }
+        if (k >= 1) {
+          const int l = k - 1;
+          const T dUdx = p_ux + ux, dUdy = p_uy + uy;
+          const T dVdx = p_vx + vx, dVdy = p_vy + vy;
+          const T rz = q * a.idz[l];
+          const T dUdz = (sU - p_su) * rz, dVdz = (sV - p_sv) * rz;
+          const T Dux = dUdx - dUdz * bx, Duy = dUdy - dUdz * by;
+          const T Dvx = dVdx - dVdz * bx, Dvy = dVdy - dVdz * by;
+          const T Exy = T(0.5) * (Dvx + Duy);
+          const T Ezz = -(Dux + Dvy);
+          const T sr2 = T(0.5) * (Dux * Dux + T(2) * (Exy * Exy) + Dvy * Dvy + Ezz * Ezz) +
+                        T(0.25) * (dUdz * dUdz + dVdz * dVdz);
+          const bool in_rng = (sr2 >= a.min_sr2) && (sr2 <= a.max_sr2);
+          const T sc = (sr2 < a.min_sr2 ? a.min_sr2 : (sr2 > a.max_sr2 ? a.max_sr2 : sr2)) + a.regu2;
+          const T pw = pow_pos(sc, a.e_glen);
+          const T wl = a.dz[l] * pw;
+          c_sh += wl * sr2;
+          // adjoint of the layer
+          const T ratio = in_rng ? a.e_glen * sr2 * rcp(sc) : T(0);
+          const T G = w * wl * (T(1) + ratio);
+          const T gDux = G * (Dux - Ezz), gDvy = G * (Dvy - Ezz), gDxy = G * Exy;
+          const T gdUdz = T(0.5) * G * dUdz - gDux * bx - gDxy * by;
+          const T gdVdz = T(0.5) * G * dVdz - gDxy * bx - gDvy * by;
+          g_ux += gDux;
+          g_uy += gDxy;
+          g_vx += gDxy;
+          g_vy += gDvy;
+          n_ux = gDux;
+          n_uy = gDxy;
+          n_vx = gDxy;
+          n_vy = gDvy;
+          const T gzu = gdUdz * rz, gzv = gdVdz * rz;
+          g_su -= gzu;
+          n_su += gzu;
+          g_sv -= gzv;
+          n_sv += gzv;
+          emit(k - 1, g_ux, g_uy, g_su, g_vx, g_vy, g_sv);
+        }
+        p_ux = ux; p_uy = uy; p_vx = vx; p_vy = vy; p_su = sU; p_sv = sV;
+        g_ux = n_ux; g_uy = n_uy; g_vx = n_vx; g_vy = n_vy; g_su = n_su; g_sv = n_sv;
+      }
+      emit(NZ - 1, g_ux, g_uy, g_su, g_vx, g_vy, g_sv);
+
+      if (emit_row) {
+        e_sh += (double)(w * c_sh);
+        e_sl += (double)c_sl;
+        e_gr += (double)c_gr;
+      }
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) {
+        up[k] = uc[k]; vp[k] = vc[k];
+        uc[k] = un[k]; vc[k] = vn[k];
+      }
+    }
+  }
+
+  // deterministic block reduction of the three energy terms
+  __shared__ double red[kWavesPerBlock][3];
+  e_sh = wave_sum(e_sh);
+  e_sl = wave_sum(e_sl);
+  e_gr = wave_sum(e_gr);
+  if (lane == 0) {
+    red[threadIdx.x >> 6][0] = e_sh;
+    red[threadIdx.x >> 6][1] = e_sl;
+    red[threadIdx.x >> 6][2] = e_gr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && !stopped) {
+    double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int wv = 0; wv < kWavesPerBlock; ++wv) {
+      s0 += red[wv][0];
+      s1 += red[wv][1];
+      s2 += red[wv][2];
+    }
+    double* o = a.partials + 3 * ((int64_t)b * a.blocks_per_batch + blockIdx.x);
+    o[0] = s0;
+    o[1] = s1;
+    o[2] = s2;
+  }
+}
+
+struct PassGeom {
+  int rows, nstrips, nrowblk, blocks;
+};
+inline PassGeom pass_geom(int Ny, int Nx) {
+  PassGeom g;
+  g.rows = kDefaultRows;
+  g.nstrips = (Nx + kStripOut - 1) / kStripOut;
+  g.nrowblk = (Ny + g.rows - 1) / g.rows;
+  const int waves = g.nstrips * g.nrowblk;
+  g.blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+  return g;
+}
+
+// ------------------------------------------------------------- finalize --
+
+constexpr int kFinBlock = 256;
+
+__global__ void __launch_bounds__(kFinBlock)
+energy_finalize_kernel(const double* __restrict__ partials, int blocks_per_batch,
+                       double* __restrict__ out, IgmEarlyStop* es, double* cost_hist, int iter) {
+  const int b = blockIdx.x;
+  if (es != nullptr && es->stopped) return;
+  __shared__ double red[kFinBlock][3];
+  double s0 = 0, s1 = 0, s2 = 0;
+  const double* p = partials + 3 * (int64_t)b * blocks_per_batch;
+  for (int i = threadIdx.x; i < blocks_per_batch; i += kFinBlock) {
+    s0 += p[3 * i];
+    s1 += p[3 * i + 1];
+    s2 += p[3 * i + 2];
+  }
+  red[threadIdx.x][0] = s0;
+  red[threadIdx.x][1] = s1;
+  red[threadIdx.x][2] = s2;
+  __syncthreads();
+  for (int h = kFinBlock / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      red[threadIdx.x][0] += red[threadIdx.x + h][0];
+      red[threadIdx.x][1] += red[threadIdx.x + h][1];
+      red[threadIdx.x][2] += red[threadIdx.x + h][2];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double sh = red[0][0], sl = red[0][1], gr = red[0][2];
+    const double cost = (sh + sl) + gr;  // tf.add_n in energy_components order
+    out[4 * b + 0] = sh;
+    out[4 * b + 1] = sl;
+    out[4 * b + 2] = gr;
+    out[4 * b + 3] = cost;
+    if (es != nullptr) {
+      if (cost_hist != nullptr) cost_hist[iter] = cost;
+      es->iters_done = iter + 1;
+      if (!es->has_best) {
+        es->best_loss = cost;
+        es->has_best = 1;
+      } else {
+        const double rel = (es->best_loss - cost) / fabs(es->best_loss);
+        if (rel > es->relative_min_delta) {
+          es->best_loss = cost;
+          es->wait = 0;
+        } else {
+          es->wait += 1;
+          if (es->wait >= es->patience) {
+            es->stopped = 1;
+            es->stop_iter = iter;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------ velocity outputs --
+
+template <typename T, int NZ>
+struct VwArr {
+  T w[NZ];
+};
+
+// ------------------------------------------------------------- dispatch --
+
+template <typename T>
+int coeffs_impl(const T* thk, const T* usurf, const T* arrh, const T* slid, const T* dX,
+                int64_t fb, int64_t fr, int64_t fc, const int32_t* extents, int nbatch, int Ny,
+                int Nx, const IgmPhysics* phys, T* coef, T* hscale, void* stream) {
+  IGM_CHECK_ARG(thk && usurf && arrh && slid && dX && coef && hscale && phys, "null pointer");
+  IGM_CHECK_ARG(nbatch >= 1 && Ny >= 2 && Nx >= 2, "need nbatch >= 1 and a grid of at least 2 x 2");
+  IGM_CHECK_ARG(Ny <= 65535, "Ny must be <= 65535");
+  IGM_CHECK_ARG(phys->force_negative_gravitational_energy == 0,
+                "force_negative_gravitational_energy is not supported");
+  const T inv_n = T(1.0 / phys->exp_glen);
+  const T p = T(1.0 + 1.0 / phys->exp_glen);
+  const T s = T(1.0 + 1.0 / phys->exp_weertman);
+  const T rho_g = T(phys->ice_density * phys->gravity_cst * 1e-6);
+  dim3 grid((Nx + 255) / 256, Ny, nbatch);
+  hipLaunchKernelGGL(energy_coeffs_kernel<T>, grid, dim3(256), 0, (hipStream_t)stream, thk,
+                     usurf, arrh, slid, dX, fb, fr, fc, extents, Ny, Nx, inv_n, T(1) / p, T(1) / s,
+                     T(phys->thr_ice_thk), rho_g, coef, hscale);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+template <typename T, int MODE>
+void launch_pass_nz(int nz, dim3 grid, hipStream_t st, const PassArgs<T>& a, bool* ok) {
+  *ok = true;
+  switch (nz) {
+#define IGM_NZ_CASE(N) \
+  case N: hipLaunchKernelGGL((energy_pass_kernel<T, N, MODE>), grid, dim3(kPassBlock), 0, st, a); break;
+    IGM_NZ_CASE(2)
+    IGM_NZ_CASE(3)
+    IGM_NZ_CASE(4)
+    IGM_NZ_CASE(5)
+    IGM_NZ_CASE(6)
+    IGM_NZ_CASE(8)
+    IGM_NZ_CASE(10)
+    IGM_NZ_CASE(12)
+    IGM_NZ_CASE(16)
+    IGM_NZ_CASE(20)
+#undef IGM_NZ_CASE
+    default: *ok = false;
+  }
+}
+
+template <typename T>
+int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T* V, int64_t bs,
+              int64_t ks, int64_t rs, int64_t cs, const T* coef, const T* hscale,
+              const double* dzeta, const IgmPhysics* phys, T* gU, T* gV, T* Uo, T* Vo, T* mU,
+              T* vU, T* mV, T* vV, const IgmAdam* adam, double* partials, const int32_t* stop,
+              void* stream) {
+  IGM_CHECK_ARG(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+  IGM_CHECK_ARG(U && V && coef && hscale && dzeta && phys && partials, "null pointer");
+  IGM_CHECK_ARG(nbatch >= 1 && Ny >= 2 && Nx >= 2, "need nbatch >= 1 and a grid of at least 2 x 2");
+  IGM_CHECK_ARG(nz >= 2 && nz <= IGM_MAX_NZ, "nz out of range");
+  IGM_CHECK_ARG(nbatch <= 65535, "nbatch must be <= 65535");
+  if (mode == 1) IGM_CHECK_ARG(gU && gV, "mode 1 needs gU and gV");
+  if (mode == 2) {
+    IGM_CHECK_ARG(Uo && Vo && mU && vU && mV && vV && adam, "mode 2 needs Uout, Vout, m, v and adam");
+    IGM_CHECK_ARG(Uo != U && Vo != V, "Uout/Vout must not alias U/V");
+  }
+  PassArgs<T> a{};
+  a.U = U; a.V = V; a.gU = gU; a.gV = gV; a.Uo = Uo; a.Vo = Vo;
+  a.mU = mU; a.vU = vU; a.mV = mV; a.vV = vV;
+  a.coef = coef; a.hscale = hscale; a.partials = partials; a.stop = stop;
+  a.bs = bs; a.ks = ks; a.rs = rs; a.cs = cs;
+  a.plane = (int64_t)Ny * Nx;
+  a.Ny = Ny; a.Nx = Nx;
+  const PassGeom g = pass_geom(Ny, Nx);
+  a.rows = g.rows; a.nstrips = g.nstrips; a.nrowblk = g.nrowblk; a.blocks_per_batch = g.blocks;
+  for (int k = 0; k < IGM_MAX_NZ; ++k) { a.dz[k] = a.idz[k] = a.wz[k] = T(0); }
+  for (int l = 0; l < nz - 1; ++l) {
+    IGM_CHECK_ARG(dzeta[l] > 0.0, "dzeta must be positive");
+    a.dz[l] = T(dzeta[l]);
+    a.idz[l] = T(1.0 / dzeta[l]);
+  }
+  for (int k = 0; k < nz; ++k) {
+    const double lo = k > 0 ? dzeta[k - 1] : 0.0;
+    const double hi = k < nz - 1 ? dzeta[k] : 0.0;
+    a.wz[k] = T(0.5 * (lo + hi));
+  }
+  const double p = 1.0 + 1.0 / phys->exp_glen;
+  const double s = 1.0 + 1.0 / phys->exp_weertman;
+  a.e_glen = T((p - 2.0) / 2.0);
+  // The clip bounds min_sr^2, max_sr^2 live in the working precision
+  // (cost_shear.py:172): in float32, 1e40 becomes +inf.
+  a.min_sr2 = T(phys->min_sr * phys->min_sr);
+  a.max_sr2 = T(phys->max_sr * phys->max_sr);
+  a.regu2 = T(phys->regu_glen * phys->regu_glen);
+  a.e_slid = T(s / 2.0 - 1.0);
+  a.s_slid = T(s);
+  a.regu_w2 = T(phys->regu_weertman * phys->regu_weertman);
+  if (adam) {
+    a.alpha = T(adam->alpha);
+    a.omb1 = T(1) - T(adam->beta1);
+    a.omb2 = T(1) - T(adam->beta2);
+    a.eps = T(adam->epsilon);
+  }
+  dim3 grid(g.blocks, nbatch);
+  bool ok = false;
+  hipStream_t st = (hipStream_t)stream;
+  if (mode == 0) launch_pass_nz<T, 0>(nz, grid, st, a, &ok);
+  else if (mode == 1) launch_pass_nz<T, 1>(nz, grid, st, a, &ok);
+  else launch_pass_nz<T, 2>(nz, grid, st, a, &ok);
+  IGM_CHECK_ARG(ok, "nz not compiled (supported: 2,3,4,5,6,8,10,12,16,20)");
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // namespace igm
+
+// The velocity-outputs kernel takes the vertical weights by value.
+namespace igm {
+
+template <typename T, int NZ>
+__global__ void __launch_bounds__(256)
+velout_kernel(int Ny, int Nx, const T* __restrict__ U, const T* __restrict__ V, int64_t ks,
+              int64_t rs, int64_t cs, const T* __restrict__ thk, VwArr<T, NZ> vw, T* Uo, T* Vo,
+              T* ub, T* vb, T* us, T* vs, T* ubar, T* vbar) {
+  const int64_t n = (int64_t)Ny * Nx;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int j = (int)(idx / Nx), i = (int)(idx % Nx);
+  const int64_t o = (int64_t)j * rs + (int64_t)i * cs;
+  const bool ice = thk[idx] > T(0);
+  T sbu = 0, sbv = 0, u0 = 0, v0 = 0, u1 = 0, v1 = 0;
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    const T u = ice ? U[o + k * ks] : T(0);
+    const T v = ice ? V[o + k * ks] : T(0);
+    if (Uo) Uo[k * n + idx] = u;
+    if (Vo) Vo[k * n + idx] = v;
+    sbu += u * vw.w[k];
+    sbv += v * vw.w[k];
+    if (k == 0) { u0 = u; v0 = v; }
+    if (k == NZ - 1) { u1 = u; v1 = v; }
+  }
+  if (ub) ub[idx] = u0;
+  if (vb) vb[idx] = v0;
+  if (us) us[idx] = u1;
+  if (vs) vs[idx] = v1;
+  if (ubar) ubar[idx] = sbu;
+  if (vbar) vbar[idx] = sbv;
+}
+
+template <typename T, int NZ>
+void velout_launch(int Ny, int Nx, const T* U, const T* V, int64_t ks, int64_t rs, int64_t cs,
+                   const T* thk, const double* vert_weight, T* Uo, T* Vo, T* ub, T* vb, T* us,
+                   T* vs, T* ubar, T* vbar, hipStream_t st) {
+  VwArr<T, NZ> vw;
+  for (int k = 0; k < NZ; ++k) vw.w[k] = T(vert_weight[k]);
+  const int64_t n = (int64_t)Ny * Nx;
+  hipLaunchKernelGGL((velout_kernel<T, NZ>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     Ny, Nx, U, V, ks, rs, cs, thk, vw, Uo, Vo, ub, vb, us, vs, ubar, vbar);
+}
+
+template <typename T>
+int velout_impl2(int nz, int Ny, int Nx, const T* U, const T* V, int64_t ks, int64_t rs,
+                 int64_t cs, const T* thk, const double* vert_weight, T* Uo, T* Vo, T* ub,
+                 T* vb, T* us, T* vs, T* ubar, T* vbar, void* stream) {
+  IGM_CHECK_ARG(U && V && thk && vert_weight, "null pointer");
+  IGM_CHECK_ARG(Ny >= 1 && Nx >= 1, "empty grid");
+  hipStream_t st = (hipStream_t)stream;
+  switch (nz) {
+#define IGM_VZ(N) \
+  case N: velout_launch<T, N>(Ny, Nx, U, V, ks, rs, cs, thk, vert_weight, Uo, Vo, ub, vb, us, vs, ubar, vbar, st); break;
+    IGM_VZ(2) IGM_VZ(3) IGM_VZ(4) IGM_VZ(5) IGM_VZ(6) IGM_VZ(8) IGM_VZ(10) IGM_VZ(12) IGM_VZ(16) IGM_VZ(20)
+#undef IGM_VZ
+    default:
+      set_error("igm_velocity_outputs: nz not compiled (supported: 2,3,4,5,6,8,10,12,16,20)");
+      return -1;
+  }
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+}  // namespace igm
+
+extern "C" {
+
+const char* igm_last_error(void) { return igm::last_error(); }
+
+int64_t igm_energy_workspace_doubles(int32_t nbatch, int32_t Ny, int32_t Nx) {
+  if (nbatch < 1 || Ny < 2 || Nx < 2) return 0;
+  const igm::PassGeom g = igm::pass_geom(Ny, Nx);
+  return 3 * (int64_t)nbatch * g.blocks;
+}
+
+int igm_energy_coeffs_f32(const float* thk, const float* usurf, const float* arrhenius,
+                          const float* slidingco, const float* dX, int64_t fb, int64_t fr,
+                          int64_t fc, const int32_t* extents, int32_t nbatch, int32_t Ny,
+                          int32_t Nx, const IgmPhysics* phys, float* coef, float* hscale,
+                          void* stream) {
+  return igm::coeffs_impl<float>(thk, usurf, arrhenius, slidingco, dX, fb, fr, fc, extents, nbatch,
+                                 Ny, Nx, phys, coef, hscale, stream);
+}
+int igm_energy_coeffs_f64(const double* thk, const double* usurf, const double* arrhenius,
+                          const double* slidingco, const double* dX, int64_t fb, int64_t fr,
+                          int64_t fc, const int32_t* extents, int32_t nbatch, int32_t Ny,
+                          int32_t Nx, const IgmPhysics* phys, double* coef, double* hscale,
+                          void* stream) {
+  return igm::coeffs_impl<double>(thk, usurf, arrhenius, slidingco, dX, fb, fr, fc, extents,
+                                  nbatch, Ny, Nx, phys, coef, hscale, stream);
+}
+
+int igm_energy_pass_f32(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
+                        const float* U, const float* V, int64_t bstride, int64_t kstride,
+                        int64_t rstride, int64_t cstride, const float* coef, const float* hscale,
+                        const double* dzeta, const IgmPhysics* phys, float* gU, float* gV,
+                        float* Uout, float* Vout, float* mU, float* vU, float* mV, float* vV,
+                        const IgmAdam* adam, double* partials, const int32_t* stop_flag,
+                        void* stream) {
+  return igm::pass_impl<float>(mode, nz, nbatch, Ny, Nx, U, V, bstride, kstride, rstride, cstride,
+                               coef, hscale, dzeta, phys, gU, gV, Uout, Vout, mU, vU, mV, vV, adam,
+                               partials, stop_flag, stream);
+}
+int igm_energy_pass_f64(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
+                        const double* U, const double* V, int64_t bstride, int64_t kstride,
+                        int64_t rstride, int64_t cstride, const double* coef,
+                        const double* hscale, const double* dzeta, const IgmPhysics* phys,
+                        double* gU, double* gV, double* Uout, double* Vout, double* mU,
+                        double* vU, double* mV, double* vV, const IgmAdam* adam,
+                        double* partials, const int32_t* stop_flag, void* stream) {
+  return igm::pass_impl<double>(mode, nz, nbatch, Ny, Nx, U, V, bstride, kstride, rstride,
+                                cstride, coef, hscale, dzeta, phys, gU, gV, Uout, Vout, mU, vU, mV,
+                                vV, adam, partials, stop_flag, stream);
+}
+
+int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int32_t Nx,
+                        double* out, IgmEarlyStop* es, double* cost_hist, int32_t iter,
+                        void* stream) {
+  IGM_CHECK_ARG(partials && out, "null pointer");
+  IGM_CHECK_ARG(nbatch >= 1 && Ny >= 2 && Nx >= 2, "bad sizes");
+  IGM_CHECK_ARG(es == nullptr || nbatch == 1, "early stopping needs nbatch == 1");
+  const igm::PassGeom g = igm::pass_geom(Ny, Nx);
+  hipLaunchKernelGGL(igm::energy_finalize_kernel, dim3(nbatch), dim3(igm::kFinBlock), 0,
+                     (hipStream_t)stream, partials, g.blocks, out, es, cost_hist, iter);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_velocity_outputs_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, const float* V,
+                             int64_t kstride, int64_t rstride, int64_t cstride, const float* thk,
+                             const double* vert_weight, float* Uo, float* Vo, float* uvelbase,
+                             float* vvelbase, float* uvelsurf, float* vvelsurf, float* ubar,
+                             float* vbar, void* stream) {
+  return igm::velout_impl2<float>(nz, Ny, Nx, U, V, kstride, rstride, cstride, thk, vert_weight, Uo,
+                                  Vo, uvelbase, vvelbase, uvelsurf, vvelsurf, ubar, vbar, stream);
+}
+int igm_velocity_outputs_f64(int32_t nz, int32_t Ny, int32_t Nx, const double* U,
+                             const double* V, int64_t kstride, int64_t rstride, int64_t cstride,
+                             const double* thk, const double* vert_weight, double* Uo, double* Vo,
+                             double* uvelbase, double* vvelbase, double* uvelsurf,
+                             double* vvelsurf, double* ubar, double* vbar, void* stream) {
+  return igm::velout_impl2<double>(nz, Ny, Nx, U, V, kstride, rstride, cstride, thk, vert_weight,
+                                   Uo, Vo, uvelbase, vvelbase, uvelsurf, vvelsurf, ubar, vbar,
+                                   stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+"""Energy operator: the fused HIP energy/adjoint pass behind a Python API.
+
+``EnergyOperator`` holds the velocity-independent per-cell coefficients of
+one problem (computed once by ``igm_energy_coeffs_*``) and evaluates the
+Blatter-Pattyn energy of (U, V) -- and its gradient -- with
+``igm_energy_pass_*`` + ``igm_energy_finalize``.  ``EnergyFunction`` wraps it
+as a ``torch.autograd.Function`` whose forward pass already computes the
+gradient (one fused sweep), so backward is a scale.
+
+Reference: igm/processes/iceflow/energy/energy.py:12-30 (iceflow_energy,
+iceflow_energy_XY), cost_shear.py, cost_gravity.py,
+cost_sliding_weertman.py; solve/solve.py:171-198.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+DTYPES = {torch.float32: "f32", torch.float64: "f64"}
+
+
+def physics_struct(phys):
+    """IgmPhysics from a cfg.processes.iceflow.physics-like object or dict."""
+    g = (lambda k: phys[k]) if isinstance(phys, dict) else (lambda k: getattr(phys, k))
+    if g("force_negative_gravitational_energy"):
+        raise NotImplementedError("force_negative_gravitational_energy is not supported")
+    return L.IgmPhysics(
+        exp_glen=float(g("exp_glen")), exp_weertman=float(g("exp_weertman")),
+        regu_glen=float(g("regu_glen")), regu_weertman=float(g("regu_weertman")),
+        thr_ice_thk=float(g("thr_ice_thk")), min_sr=float(g("min_sr")), max_sr=float(g("max_sr")),
+        ice_density=float(g("ice_density")), gravity_cst=float(g("gravity_cst")),
+        force_negative_gravitational_energy=0)
+
+
+def velocity_strides(U, V=None, nz=None):
+    """(U ptr, V ptr, bstride, kstride, rstride, cstride) of a velocity pair.
+
+    Accepts planar U, V of shape (Nz, Ny, Nx) or (B, Nz, Ny, Nx), or an
+    emulator output Y of shape (B, Ny, Nx, 2Nz) (V=None, nz given).
+    """
+    if V is None:
+        Y = U
+        assert Y.dim() == 4 and nz is not None and Y.shape[-1] == 2 * nz
+        es = Y.element_size()
+        return (Y.data_ptr(), Y.data_ptr() + nz * es * Y.stride(3), Y.stride(0), Y.stride(3),
+                Y.stride(1), Y.stride(2))
+    assert U.shape == V.shape and U.stride() == V.stride()
+    if U.dim() == 3:
+        return (U.data_ptr(), V.data_ptr(), 0, U.stride(0), U.stride(1), U.stride(2))
+    assert U.dim() == 4
+    return (U.data_ptr(), V.data_ptr(), U.stride(0), U.stride(1), U.stride(2), U.stride(3))
+
+
+class EnergyOperator:
+    """Coefficients + evaluation of the energy for a fixed set of input fields.
+
+    fields: dict with thk, usurf, arrhenius, slidingco, dX as device tensors
+    of shape (Ny, Nx) or (B, Ny, Nx); or ``X`` NHWC (B, Ny, Nx, 5) in the
+    order [thk, usurf, arrhenius, slidingco, dX] (iceflow/utils.py:147-158).
+    extents: optional int32 device tensor (B, 2) of valid (ny, nx) per entry.
+    """
+
+    def __init__(self, dzeta, phys, fields=None, X=None, extents=None, dtype=torch.float32):
+        self.dtype = dtype
+        self.sfx = DTYPES[dtype]
+        self.nz = len(dzeta) + 1
+        self.dzeta = (C.c_double * max(1, len(dzeta)))(*[float(d) for d in dzeta])
+        self.phys = physics_struct(phys)
+        if X is not None:
+            assert X.dim() == 4 and X.shape[-1] == 5 and X.dtype == dtype and X.is_contiguous()
+            B, Ny, Nx, _ = X.shape
+            es = X.element_size()
+            base = X.data_ptr()
+            ptrs = [base + f * es for f in range(5)]
+            fb, fr, fc = X.stride(0), X.stride(1), X.stride(2)
+            self._keep = X
+        else:
+            fl = [fields[k] for k in ("thk", "usurf", "arrhenius", "slidingco", "dX")]
+            t0 = fl[0]
+            if t0.dim() == 2:
+                fl = [f[None] for f in fl]
+            fl = [f.to(dtype).contiguous() for f in fl]
+            B, Ny, Nx = fl[0].shape
+            ptrs = [f.data_ptr() for f in fl]
+            fb, fr, fc = fl[0].stride()
+            self._keep = fl
+        self.B, self.Ny, self.Nx = B, Ny, Nx
+        dev = self._keep[0].device if isinstance(self._keep, list) else self._keep.device
+        self.device = dev
+        self.extents = extents
+        self.coef = torch.empty((B, 7, Ny, Nx), dtype=dtype, device=dev)
+        self.hscale = torch.empty((B,), dtype=dtype, device=dev)
+        nws = L.load().igm_energy_workspace_doubles(B, Ny, Nx)
+        self.partials = torch.empty((max(1, nws),), dtype=torch.float64, device=dev)
+        self.out = torch.zeros((B, 4), dtype=torch.float64, device=dev)
+        L.call(f"igm_energy_coeffs_{self.sfx}", *ptrs, fb, fr, fc, L.ptr(extents), B, Ny, Nx,
+               C.addressof(self.phys), L.ptr(self.coef), L.ptr(self.hscale), L.stream_ptr())
+
+    # -- raw passes ---------------------------------------------------------
+    def run_pass(self, mode, U, V=None, gU=None, gV=None, Uo=None, Vo=None, mU=None, vU=None,
+                 mV=None, vV=None, adam=None, stop_flag=None):
+        up, vp, bs, ks, rs, cs = velocity_strides(U, V, self.nz)
+        if gU is not None and gV is None:  # gradient in Y layout
+            gu, gv = velocity_strides(gU, None, self.nz)[:2]
+        else:
+            gu, gv = L.ptr(gU), L.ptr(gV)
+        L.call(f"igm_energy_pass_{self.sfx}", mode, self.nz, self.B, self.Ny, self.Nx, up, vp,
+               bs, ks, rs, cs, L.ptr(self.coef), L.ptr(self.hscale), C.addressof(self.dzeta),
+               C.addressof(self.phys), gu, gv, L.ptr(Uo), L.ptr(Vo), L.ptr(mU), L.ptr(vU),
+               L.ptr(mV), L.ptr(vV), C.addressof(adam) if adam is not None else None,
+               L.ptr(self.partials), stop_flag, L.stream_ptr())
+
+    def finalize(self, out=None, es=None, cost_hist=None, it=0):
+        out = self.out if out is None else out
+        L.call("igm_energy_finalize", L.ptr(self.partials), self.B, self.Ny, self.Nx, L.ptr(out),
+               es, L.ptr(cost_hist), it, L.stream_ptr())
+        return out
+
+    # -- convenience --------------------------------------------------------
+    def energy(self, U, V=None):
+        """Per-entry [mean shear, mean sliding, mean gravity, COST] (float64, device)."""
+        self.run_pass(0, U, V)
+        return self.finalize().clone()
+
+    def energy_grad(self, U, V=None):
+        """(energies (B,4) float64, gU, gV) -- or (energies, gY) for a Y input."""
+        if V is None:
+            gY = torch.empty_like(U)
+            self.run_pass(1, U, None, gU=gY)
+            return self.finalize().clone(), gY
+        gU = torch.empty_like(U)
+        gV = torch.empty_like(V)
+        self.run_pass(1, U, V, gU=gU, gV=gV)
+        return self.finalize().clone(), gU, gV
+
+
+class EnergyFunction(torch.autograd.Function):
+    """J(Y) = sum over batch entries of COST_b, with dJ/dY from the fused pass.
+
+    Y: emulator output (B, Ny, Nx, 2Nz) float32.  Returns a float64 scalar.
+    The reference sums the per-patch costs of a retraining epoch
+    (emulate/emulate.py:222-226); each COST_b is already a mean over cells.
+    """
+
+    @staticmethod
+    def forward(ctx, Y, op):
+        Y = Y.contiguous()
+        e, gY = op.energy_grad(Y)
+        ctx.save_for_backward(gY)
+        return e[:, 3].sum()
+
+    @staticmethod
+    def backward(ctx, g):
+        (gY,) = ctx.saved_tensors
+        return gY * g.to(gY.dtype), None

@@ -1,0 +1,94 @@
+"""Plain energy solver on the GPU: gradient descent with legacy Adam.
+
+Restates igm/processes/iceflow/solve/solve.py:132-240 MI355X-first:
+
+* one fused kernel per iteration evaluates the energy, its adjoint and the
+  Adam update (ping-pong velocity buffers, moments updated in place);
+* a tiny float64 reduction kernel per iteration produces COST and runs the
+  EarlyStopping rule (iceflow/utils.py:174-204) on the device;
+* the host never waits on a COST value inside the loop: it reads the
+  device stop flag every ``check_every`` iterations.  Iterations enqueued
+  after the stop are no-ops (the kernels test the flag), so the result is
+  the one the reference's per-iteration ``COST.numpy()`` check gives.
+
+Adam step constants follow TensorFlow's legacy Adam (see
+``LegacyAdamState``); the step counter persists across solves like the
+optimizer created once by ``initialize_iceflow_solver`` (solve.py:132-141).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .energy import EnergyOperator
+
+
+class LegacyAdamState:
+    """Global step counter + hyper-parameters of tf.keras.optimizers.legacy.Adam."""
+
+    def __init__(self, lr=1.0, beta1=0.9, beta2=0.999, epsilon=1e-7):
+        self.lr, self.beta1, self.beta2, self.epsilon = lr, beta1, beta2, epsilon
+        self.iterations = 0
+
+    def alpha(self, t, dtype=torch.float32):
+        """lr * sqrt(1 - beta2^t) / (1 - beta1^t) at step t (1-based), in the
+        working precision (float32 for the float32 solver, as TF computes it)."""
+        if dtype == torch.float32:
+            f = np.float32
+            b1p = f(np.power(f(self.beta1), f(t)))
+            b2p = f(np.power(f(self.beta2), f(t)))
+            return float(f(self.lr) * np.sqrt(f(1) - b2p) / (f(1) - b1p))
+        return self.lr * np.sqrt(1 - self.beta2 ** t) / (1 - self.beta1 ** t)
+
+
+class SolveResult:
+    def __init__(self, U, V, costs, iterations, stopped):
+        self.U, self.V, self.costs, self.iterations, self.stopped = U, V, costs, iterations, stopped
+
+
+def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early_stopping=True,
+          relative_min_delta=0.0002, patience=10, check_every=16, lr_schedule=None):
+    """Minimise the energy of ``op`` from (U0, V0) (planar (Nz, Ny, Nx)).
+
+    Returns SolveResult with unmasked U, V (device), the COST history (host
+    float64 numpy array) and the number of iterations whose update was
+    applied.  ``lr_schedule(i)`` (optional) overrides the learning rate of
+    iteration i (the reference uses a constant step_size).
+    """
+    dt = op.dtype
+    assert U0.dtype == dt and U0.shape == V0.shape and U0.dim() == 3
+    dev = U0.device
+    Ub = [U0.detach().clone().contiguous(), torch.empty_like(U0)]
+    Vb = [V0.detach().clone().contiguous(), torch.empty_like(V0)]
+    mU, vU, mV, vV = (torch.zeros_like(Ub[0]) for _ in range(4))
+    es_host = L.IgmEarlyStop(best_loss=0.0, relative_min_delta=relative_min_delta, has_best=0,
+                             wait=0, patience=patience if early_stopping else 2**30, stopped=0,
+                             stop_iter=-1, iters_done=0)
+    es_dev = torch.frombuffer(bytearray(bytes(es_host)), dtype=torch.uint8).to(dev)
+    es_ptr = es_dev.data_ptr()
+    stop_ptr = es_ptr + L.EARLYSTOP_STOPPED_OFFSET
+    hist = torch.zeros(max(1, nbitmax), dtype=torch.float64, device=dev)
+    adam = L.IgmAdam(alpha=0.0, beta1=optimizer.beta1, beta2=optimizer.beta2,
+                     epsilon=optimizer.epsilon)
+    t0 = optimizer.iterations
+    lr0 = optimizer.lr
+    for i in range(nbitmax):
+        if lr_schedule is not None:
+            optimizer.lr = lr_schedule(i)
+        adam.alpha = optimizer.alpha(t0 + i + 1, dt)
+        src, dst = i % 2, (i + 1) % 2
+        op.run_pass(2, Ub[src], Vb[src], Uo=Ub[dst], Vo=Vb[dst], mU=mU, vU=vU, mV=mV, vV=vV,
+                    adam=adam, stop_flag=stop_ptr)
+        op.finalize(es=es_ptr, cost_hist=hist, it=i)
+        if early_stopping and (i + 1) % check_every == 0:
+            if int(es_dev[L.EARLYSTOP_STOPPED_OFFSET:L.EARLYSTOP_STOPPED_OFFSET + 4]
+                   .view(torch.int32).item()) != 0:
+                break
+    optimizer.lr = lr0
+    st = L.IgmEarlyStop.from_buffer_copy(bytes(es_dev.cpu().numpy().tobytes()))
+    n = int(st.iters_done)
+    optimizer.iterations = t0 + n
+    res = n % 2
+    costs = hist[:n].cpu().numpy()
+    return SolveResult(Ub[res], Vb[res], costs, n, bool(st.stopped))
