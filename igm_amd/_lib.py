@@ -55,6 +55,17 @@ _SIGS = {
                                            VP, VP, VP, VP]),
     "igm_velocity_outputs_f64": (C.c_int, [I32, I32, I32, VP, VP, I64, I64, I64, VP, VP, VP, VP, VP, VP, VP,
                                            VP, VP, VP, VP]),
+    "igm_conv_in_fwd": (C.c_int, [VP, VP, VP, VP, VP, I64, I64, I64, VP, I32, I32, I32, VP, VP, VP, VP]),
+    "igm_conv3x3_fwd_bf16": (C.c_int, [VP, VP, VP, VP, I32, I32, I32, VP, VP]),
+    "igm_conv3x3_dgrad_bf16": (C.c_int, [VP, VP, VP, VP, I32, I32, I32, VP, VP]),
+    "igm_conv1x1_head_fwd": (C.c_int, [VP, VP, VP, VP, I32, I32, I32, I32, VP, VP]),
+    "igm_conv1x1_head_bwd": (C.c_int, [VP, VP, VP, VP, I32, I32, I32, I32, VP, VP, VP]),
+    "igm_wgrad_workspace_floats": (I64, [I32, I32]),
+    "igm_conv_wgrad_bf16": (C.c_int, [I32, VP, VP, VP, I32, I32, I32, I32, I32, VP, VP, VP, VP]),
+    "igm_conv_in_wgrad": (C.c_int, [VP, VP, VP, VP, VP, I64, I64, I64, VP, I32, I32, I32, VP, I32, VP, VP,
+                                    VP, VP]),
+    "igm_adam_clip_f32": (C.c_int, [VP, VP, VP, VP, I64, VP, I32, VP, F64, VP, VP]),
+    "igm_pack_conv_weights": (C.c_int, [VP, VP, I32, VP, I32, VP, VP, VP, VP]),
 }
 
 _lib = None

@@ -1,0 +1,746 @@
+// Convolutional emulator of IGM's ice flow on gfx950: forward and backward
+// kernels of the "cnn" network (emulate/neural_network.py:9-66):
+//   16 x [Conv2D 3x3 "same" + bias -> LeakyReLU(0.01)] -> Conv2D 1x1 + bias.
+//
+// Data layout: activations are NHWC bf16 (B, H, W, 32); the network output Y
+// is NHWC fp32 (B, H, W, 2Nz).  Per-batch (ny, nx) extents mask padded
+// batches: every kernel writes zeros outside a batch entry's extent, which
+// makes a padded batch behave exactly like per-entry "same" zero padding.
+//
+// Kernels:
+//  conv_in_kernel       layer 1 (5 -> 32), fp32 VALU (raw inputs such as the
+//                       surface elevation need fp32: bf16 spacing is 16 m at
+//                       2500-3000 m), bias + LeakyReLU, bf16 out.
+//  conv3x3_mfma_kernel  32 -> 32 3x3 layers on v_mfma_f32_16x16x32_bf16:
+//                       forward (bias + LeakyReLU) and dgrad (same kernel,
+//                       flipped/transposed weights, output gated by the
+//                       LeakyReLU derivative of the layer below).  Input
+//                       rows are staged in LDS (pixel stride 80 B: the
+//                       16-B fragment reads of 8 lanes hit distinct banks).
+//  head_mfma_kernel     1x1 32 -> 2Nz (<= 32) output layer, fp32 out.
+//  wgrad_mfma_kernel    weight gradient: K = pixels; tiles are transposed
+//                       into LDS as [channel][pixel] (three x-shifted copies
+//                       of the activation rows keep every fragment read
+//                       16-B aligned); per-block partial sums, then a
+//                       deterministic reduction (wgrad_reduce_kernel).
+//  head_bwd_kernel      dY -> dA16 * LeakyReLU'(A16) (fp32 VALU) and a padded
+//                       bf16 copy of dY for the head's wgrad.
+//  conv_in_wgrad_kernel layer-1 weight gradient, fp32 VALU.
+//  Adam: per-tensor clip_by_norm + legacy Adam on the fp32 master weights,
+//  then bf16 packing of the MFMA operands.
+#include "common.hpp"
+#include "../../include/igm_amd.h"
+
+namespace igm {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kC = 32;        // hidden channels
+constexpr int kCin = 5;       // input fields
+constexpr int kTileW = 64;    // pixels per tile row
+constexpr int kTileH = 4;     // output rows per forward tile (one per wave)
+constexpr int kPxStride = 40; // LDS pixel stride in bf16 (80 B)
+constexpr int kChStride = 72; // LDS channel-row stride in bf16 (144 B) for wgrad
+
+__device__ __forceinline__ float leaky(float x) { return x > 0.f ? x : 0.01f * x; }
+__device__ __forceinline__ float bf2f(bf16 h) { return (float)h; }
+__device__ __forceinline__ bf16 f2bf(float f) { return (bf16)f; }
+
+__device__ __forceinline__ void extent_of(const int32_t* ext, int b, int H, int W, int& ny, int& nx) {
+  ny = ext ? min(ext[2 * b], H) : H;
+  nx = ext ? min(ext[2 * b + 1], W) : W;
+}
+
+// ------------------------------------------------------------ layer 1 --
+// One thread per output pixel; weights HWIO [3][3][5][32] fp32 in LDS.
+__global__ void __launch_bounds__(256)
+conv_in_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
+               const float* __restrict__ f2, const float* __restrict__ f3,
+               const float* __restrict__ f4, int64_t fb, int64_t fr, int64_t fc,
+               const int32_t* __restrict__ ext, int H, int W, const float* __restrict__ w,
+               const float* __restrict__ bias, bf16* __restrict__ out) {
+  __shared__ float ws[9 * kCin * kC];
+  __shared__ float bs[kC];
+  for (int i = threadIdx.x; i < 9 * kCin * kC; i += blockDim.x) ws[i] = w[i];
+  if (threadIdx.x < kC) bs[threadIdx.x] = bias[threadIdx.x];
+  __syncthreads();
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  const int b = blockIdx.z;
+  if (x >= W) return;
+  int ny, nx;
+  extent_of(ext, b, H, W, ny, nx);
+  bf16* o = out + (((int64_t)b * H + y) * W + x) * kC;
+  if (y >= ny || x >= nx) {
+    uint4 z = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) reinterpret_cast<uint4*>(o)[q] = z;
+    return;
+  }
+  const float* fs[kCin] = {f0, f1, f2, f3, f4};
+  float acc[kC];
+#pragma unroll
+  for (int c = 0; c < kC; ++c) acc[c] = bs[c];
+#pragma unroll 1
+  for (int r = 0; r < 3; ++r) {
+    const int yy = y + r - 1;
+#pragma unroll 1
+    for (int s = 0; s < 3; ++s) {
+      const int xx = x + s - 1;
+      const bool in = yy >= 0 && yy < ny && xx >= 0 && xx < nx;
+      const int64_t off = (int64_t)b * fb + (int64_t)yy * fr + (int64_t)xx * fc;
+#pragma unroll
+      for (int ci = 0; ci < kCin; ++ci) {
+        const float v = in ? fs[ci][off] : 0.f;
+        const float* wr = ws + ((r * 3 + s) * kCin + ci) * kC;
+#pragma unroll
+        for (int c = 0; c < kC; ++c) acc[c] += v * wr[c];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = f2bf(leaky(acc[q * 8 + e]));
+    reinterpret_cast<bf16x8*>(o)[q] = v;
+  }
+}
+
+// ------------------------------------------------ 3x3 MFMA conv (fwd/dgrad) --
+// wt: [9][32 out][32 in] bf16.  MODE 0: out = LeakyReLU(conv + bias).
+// MODE 1 (dgrad): out = conv * (aux > 0 ? 1 : 0.01), aux = activation of the
+// layer below (its sign is the sign of its pre-activation).
+// Block = 4 waves; tile = 4 output rows x 64 pixels; wave w owns row y0 + w.
+template <int MODE>
+__global__ void __launch_bounds__(256)
+conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
+                    const float* __restrict__ bias, const bf16* __restrict__ aux,
+                    const int32_t* __restrict__ ext, int H, int W, bf16* __restrict__ out) {
+  __shared__ __attribute__((aligned(16))) bf16 tile[kTileH + 2][kTileW + 2][kPxStride];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int x0 = blockIdx.x * kTileW, y0 = blockIdx.y * kTileH, b = blockIdx.z;
+  int ny, nx;
+  extent_of(ext, b, H, W, ny, nx);
+  const int64_t img = (int64_t)b * H * W;
+  const int y = y0 + wave;
+
+  if (y0 >= ny || x0 >= nx) {  // tile outside the valid region: zeros
+    if (y < H) {
+      const int x = x0 + lane;
+      if (x < W) {
+        uint4 z = {0, 0, 0, 0};
+        uint4* o = reinterpret_cast<uint4*>(out + ((img + (int64_t)y * W + x) * kC));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) o[q] = z;
+      }
+    }
+    return;
+  }
+
+  // weight fragments: A operand (16 out-channels x 32 in-channels per tap)
+  bf16x8 wf[9][2];
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+      wf[t][cb] = *reinterpret_cast<const bf16x8*>(wt + ((t * kC + cb * 16 + (lane & 15)) * kC + 8 * (lane >> 4)));
+
+  // stage input rows y0-1 .. y0+4, pixels x0-1 .. x0+64 (zero outside)
+  constexpr int kChunks = (kTileH + 2) * (kTileW + 2) * 4;
+  for (int c = tid; c < kChunks; c += 256) {
+    const int chunk = c & 3;
+    const int px = (c >> 2) % (kTileW + 2);
+    const int row = (c >> 2) / (kTileW + 2);
+    const int yy = y0 - 1 + row, xx = x0 - 1 + px;
+    uint4 v = {0, 0, 0, 0};
+    if (yy >= 0 && yy < ny && xx >= 0 && xx < nx)
+      v = *reinterpret_cast<const uint4*>(in + (img + (int64_t)yy * W + xx) * kC + chunk * 8);
+    *reinterpret_cast<uint4*>(&tile[row][px][chunk * 8]) = v;
+  }
+  __syncthreads();
+
+  f32x4 acc[4][2];
+#pragma unroll
+  for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) acc[pb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+  for (int pb = 0; pb < 4; ++pb) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int r = t / 3, s = t % 3;
+      const bf16x8 bfr =
+          *reinterpret_cast<const bf16x8*>(&tile[wave + r][pb * 16 + (lane & 15) + s][8 * (lane >> 4)]);
+      acc[pb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[t][0], bfr, acc[pb][0], 0, 0, 0);
+      acc[pb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[t][1], bfr, acc[pb][1], 0, 0, 0);
+    }
+  }
+
+  if (y >= H) return;
+#pragma unroll
+  for (int pb = 0; pb < 4; ++pb) {
+    const int x = x0 + pb * 16 + (lane & 15);
+    if (x >= W) continue;
+    const bool valid = (y < ny) && (x < nx);
+    const int64_t po = (img + (int64_t)y * W + x) * kC;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int co = cb * 16 + 4 * (lane >> 4);
+      float v[4];
+      if constexpr (MODE == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = leaky(acc[pb][cb][e] + bias[co + e]);
+      } else {
+        const uint2 g2 = *reinterpret_cast<const uint2*>(aux + po + co);
+        const bf16* g = reinterpret_cast<const bf16*>(&g2);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = acc[pb][cb][e] * (bf2f(g[e]) > 0.f ? 1.f : 0.01f);
+      }
+      bf16 h[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) h[e] = valid ? f2bf(v[e]) : f2bf(0.f);
+      *reinterpret_cast<uint2*>(out + po + co) = *reinterpret_cast<const uint2*>(h);
+    }
+  }
+}
+
+// ------------------------------------------------------- 1x1 head (MFMA) --
+// wt: [32 out (zero-padded beyond cout)][32 in] bf16; Y fp32 (B,H,W,cout).
+__global__ void __launch_bounds__(256)
+head_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
+                 const float* __restrict__ bias, const int32_t* __restrict__ ext, int H, int W,
+                 int cout, float* __restrict__ Y) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t npx = (int64_t)H * W;
+  const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 64;  // 64 pixels per wave
+  const int b = blockIdx.y;
+  if (p0 >= npx) return;
+  int ny, nx;
+  extent_of(ext, b, H, W, ny, nx);
+  bf16x8 wf[2];
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb)
+    wf[cb] = *reinterpret_cast<const bf16x8*>(wt + ((cb * 16 + (lane & 15)) * kC + 8 * (lane >> 4)));
+  const bf16* ib = in + (int64_t)b * npx * kC;
+  float* Yb = Y + (int64_t)b * npx * cout;
+#pragma unroll
+  for (int pb = 0; pb < 4; ++pb) {
+    const int64_t p = p0 + pb * 16 + (lane & 15);
+    f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+    for (int kc = 0; kc < 1; ++kc) {
+      bf16x8 bfr;
+      if (p < npx) bfr = *reinterpret_cast<const bf16x8*>(ib + p * kC + 8 * (lane >> 4));
+      else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bfr[e] = f2bf(0.f);
+      }
+      acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[0], bfr, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[1], bfr, acc[1], 0, 0, 0);
+    }
+    if (p >= npx) continue;
+    const int yy = (int)(p / W), xx = (int)(p % W);
+    const bool valid = yy < ny && xx < nx;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int co = cb * 16 + 4 * (lane >> 4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (co + e < cout) Yb[p * cout + co + e] = valid ? acc[cb][e] + bias[co + e] : 0.f;
+    }
+  }
+}
+
+// ------------------------------------------------------- head backward --
+// dZ16 = (dY W17^T) * LeakyReLU'(A16) (bf16); dYb = bf16(dY) zero-padded to 32.
+// w17: HWIO [32 in][cout] fp32.
+__global__ void __launch_bounds__(256)
+head_bwd_kernel(const float* __restrict__ dY, const bf16* __restrict__ a16,
+                const float* __restrict__ w17, const int32_t* __restrict__ ext, int H, int W,
+                int cout, bf16* __restrict__ dz16, bf16* __restrict__ dYb) {
+  __shared__ float ws[kC * kC];
+  for (int i = threadIdx.x; i < kC * cout; i += blockDim.x) ws[i] = w17[i];
+  __syncthreads();
+  const int64_t npx = (int64_t)H * W;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (p >= npx) return;
+  int ny, nx;
+  extent_of(ext, b, H, W, ny, nx);
+  const int yy = (int)(p / W), xx = (int)(p % W);
+  const bool valid = yy < ny && xx < nx;
+  const int64_t q = (int64_t)b * npx + p;
+  float d[kC];
+#pragma unroll
+  for (int c = 0; c < kC; ++c) d[c] = (c < cout && valid) ? dY[q * cout + c] : 0.f;
+  bf16x8 outv[4], outy[4];
+#pragma unroll
+  for (int c4 = 0; c4 < 4; ++c4) {
+    const bf16x8 av = reinterpret_cast<const bf16x8*>(a16 + q * kC)[c4];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int ci = c4 * 8 + e;
+      float s = 0.f;
+      for (int co = 0; co < cout; ++co) s += d[co] * ws[ci * cout + co];
+      s *= (bf2f(av[e]) > 0.f ? 1.f : 0.01f);
+      outv[c4][e] = f2bf(valid ? s : 0.f);
+      outy[c4][e] = f2bf(d[ci]);
+    }
+  }
+#pragma unroll
+  for (int c4 = 0; c4 < 4; ++c4) {
+    reinterpret_cast<bf16x8*>(dz16 + q * kC)[c4] = outv[c4];
+    reinterpret_cast<bf16x8*>(dYb + q * kC)[c4] = outy[c4];
+  }
+}
+
+// ------------------------------------------------------------ wgrad MFMA --
+// dW[tap][co][ci] = sum_p A[p + delta_tap][ci] * dZ[p][co],  db[co] = sum_p dZ[p][co].
+// Persistent blocks over tiles of 1 row x 64 pixels; TAPS = 9 (3x3) or 1 (1x1).
+// Output blocks (tap, co-block, ci-block) of 16x16 are split over the 4 waves.
+// partials: [gridDim.x][TAPS*1024 + 32] fp32.
+template <int TAPS>
+__global__ void __launch_bounds__(256)
+wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
+                  const int32_t* __restrict__ ext, int B, int H, int W, float* __restrict__ partials) {
+  constexpr int ROWS = TAPS == 9 ? 3 : 1;
+  constexpr int SHIFTS = TAPS == 9 ? 3 : 1;
+  constexpr int NBLK = TAPS * 4;
+  constexpr int PER_WAVE = NBLK / 4;
+  __shared__ __attribute__((aligned(16))) bf16 lds_dz[kC][kChStride];
+  __shared__ __attribute__((aligned(16))) bf16 lds_act[ROWS * SHIFTS][kC][kChStride];
+  __shared__ float dbred[256][8];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ntx = (W + kTileW - 1) / kTileW;
+  const int64_t ntiles = (int64_t)B * H * ntx;
+
+  f32x4 acc[PER_WAVE];
+#pragma unroll
+  for (int i = 0; i < PER_WAVE; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float dba[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) dba[e] = 0.f;
+
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int b = (int)(tile / ((int64_t)H * ntx));
+    const int y = (int)((tile / ntx) % H);
+    const int x0 = (int)(tile % ntx) * kTileW;
+    int ny, nx;
+    extent_of(ext, b, H, W, ny, nx);
+    if (y >= ny || x0 >= nx) continue;  // block-uniform
+    const int64_t img = (int64_t)b * H * W;
+    __syncthreads();
+    {  // dZ row y, pixels x0 .. x0+63 -> lds_dz[ch][px]
+      const int px = tid >> 2, chunk = tid & 3;
+      const int x = x0 + px;
+      bf16x8 v;
+      if (x < nx) v = *reinterpret_cast<const bf16x8*>(dz + (img + (int64_t)y * W + x) * kC + chunk * 8);
+      else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = f2bf(0.f);
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        lds_dz[chunk * 8 + e][px] = v[e];
+        dba[e] += bf2f(v[e]);
+      }
+    }
+    // activation rows (y-1 .. y+1 for 3x3), pixels x0-1 .. x0+64, x-shifted copies
+    constexpr int kChunks = ROWS * (kTileW + 2) * 4;
+    for (int c = tid; c < kChunks; c += 256) {
+      const int chunk = c & 3;
+      const int t = (c >> 2) % (kTileW + 2);
+      const int row = (c >> 2) / (kTileW + 2);
+      const int yy = (ROWS == 3) ? y - 1 + row : y;
+      const int xx = x0 - 1 + t;
+      bf16x8 v;
+      if (yy >= 0 && yy < ny && xx >= 0 && xx < nx)
+        v = *reinterpret_cast<const bf16x8*>(act + (img + (int64_t)yy * W + xx) * kC + chunk * 8);
+      else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = f2bf(0.f);
+      }
+#pragma unroll
+      for (int s = 0; s < SHIFTS; ++s) {
+        const int p = t - ((SHIFTS == 3) ? s : 1);
+        if (p >= 0 && p < kTileW) {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) lds_act[row * SHIFTS + s][chunk * 8 + e][p] = v[e];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER_WAVE; ++i) {
+      const int blk = wave * PER_WAVE + i;
+      const int tap = blk >> 2, cb = (blk >> 1) & 1, cib = blk & 1;
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&lds_dz[cb * 16 + (lane & 15)][ks * 32 + 8 * (lane >> 4)]);
+        const bf16x8 bb = *reinterpret_cast<const bf16x8*>(&lds_act[tap][cib * 16 + (lane & 15)][ks * 32 + 8 * (lane >> 4)]);
+        acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[i], 0, 0, 0);
+      }
+    }
+  }
+  // partial dW: D[co][ci], lane: ci = cib*16 + lane%16, co = cb*16 + 4*(lane/16) + e
+  float* P = partials + (int64_t)blockIdx.x * (TAPS * kC * kC + kC);
+#pragma unroll
+  for (int i = 0; i < PER_WAVE; ++i) {
+    const int blk = wave * PER_WAVE + i;
+    const int tap = blk >> 2, cb = (blk >> 1) & 1, cib = blk & 1;
+    const int ci = cib * 16 + (lane & 15);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int co = cb * 16 + 4 * (lane >> 4) + e;
+      P[(tap * kC + co) * kC + ci] = acc[i][e];
+    }
+  }
+  // partial db: threads with the same (tid & 3) hold channels (tid&3)*8 .. +8
+#pragma unroll
+  for (int e = 0; e < 8; ++e) dbred[tid][e] = dba[e];
+  __syncthreads();
+  if (tid < kC) {
+    const int chunk = tid >> 3, e = tid & 7;
+    float s = 0.f;
+    for (int t = chunk; t < 256; t += 4) s += dbred[t][e];
+    P[TAPS * kC * kC + tid] = s;
+  }
+}
+
+// out_hwio[tap][ci][co] = sum_blk P[blk][(tap*32 + co)*32 + ci] (co < cout);
+// db[co] = sum_blk P[blk][TAPS*1024 + co].
+__global__ void __launch_bounds__(256)
+wgrad_reduce_kernel(const float* __restrict__ P, int nblk, int taps, int cout,
+                    float* __restrict__ dW, float* __restrict__ db) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nw = taps * kC * cout;
+  const int64_t stride = (int64_t)taps * kC * kC + kC;
+  if (i < nw) {
+    const int co = i % cout, ci = (i / cout) % kC, tap = i / (cout * kC);
+    const int64_t src = (tap * kC + co) * kC + ci;
+    float s = 0.f;
+    for (int k = 0; k < nblk; ++k) s += P[k * stride + src];
+    dW[i] = s;
+  } else if (i < nw + cout) {
+    const int co = i - nw;
+    float s = 0.f;
+    for (int k = 0; k < nblk; ++k) s += P[k * stride + (int64_t)taps * kC * kC + co];
+    db[co] = s;
+  }
+}
+
+// ------------------------------------------------------ layer-1 wgrad --
+// dW1[r][s][ci][co] (HWIO, fp32) = sum_p X[p + delta][ci] * dZ1[p][co]; db1.
+// Tiles of 1 row x 256 pixels; thread t: co = t % 32, owns pairs
+// (tap*5 + ci) = g, g+8, ... (g = t / 32).  partials: [grid][1440 + 32].
+constexpr int kInTile = 256;
+__global__ void __launch_bounds__(256)
+conv_in_wgrad_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
+                     const float* __restrict__ f2, const float* __restrict__ f3,
+                     const float* __restrict__ f4, int64_t fb, int64_t fr, int64_t fc,
+                     const int32_t* __restrict__ ext, int B, int H, int W,
+                     const bf16* __restrict__ dz, float* __restrict__ partials) {
+  __shared__ float xs[3][kInTile + 2][kCin];
+  __shared__ float dzs[kInTile][kC + 1];
+  const int tid = threadIdx.x;
+  const int co = tid & 31, g = tid >> 5;
+  const float* fs[kCin] = {f0, f1, f2, f3, f4};
+  float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float dbacc = 0.f;
+  const int ntx = (W + kInTile - 1) / kInTile;
+  const int64_t ntiles = (int64_t)B * H * ntx;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int b = (int)(tile / ((int64_t)H * ntx));
+    const int y = (int)((tile / ntx) % H);
+    const int x0 = (int)(tile % ntx) * kInTile;
+    int ny, nx;
+    extent_of(ext, b, H, W, ny, nx);
+    if (y >= ny || x0 >= nx) continue;
+    __syncthreads();
+    for (int c = tid; c < 3 * (kInTile + 2) * kCin; c += 256) {
+      const int ci = c % kCin, t = (c / kCin) % (kInTile + 2), row = c / (kCin * (kInTile + 2));
+      const int yy = y - 1 + row, xx = x0 - 1 + t;
+      float v = 0.f;
+      if (yy >= 0 && yy < ny && xx >= 0 && xx < nx)
+        v = fs[ci][(int64_t)b * fb + (int64_t)yy * fr + (int64_t)xx * fc];
+      xs[row][t][ci] = v;
+    }
+    for (int c = tid; c < kInTile * 4; c += 256) {
+      const int px = c >> 2, chunk = c & 3;
+      const int x = x0 + px;
+      bf16x8 v;
+      if (x < nx) v = *reinterpret_cast<const bf16x8*>(dz + (((int64_t)b * H + y) * W + x) * kC + chunk * 8);
+      else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = f2bf(0.f);
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) dzs[px][chunk * 8 + e] = bf2f(v[e]);
+    }
+    __syncthreads();
+    for (int px = 0; px < kInTile; ++px) {
+      const float d = dzs[px][co];
+      dbacc += d;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const int pair = g + 8 * j;
+        if (pair < 9 * kCin) {
+          const int tap = pair / kCin, ci = pair % kCin;
+          acc[j] += xs[tap / 3][px + tap % 3][ci] * d;
+        }
+      }
+    }
+  }
+  float* P = partials + (int64_t)blockIdx.x * (9 * kCin * kC + kC);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const int pair = g + 8 * j;
+    if (pair < 9 * kCin) P[pair * kC + co] = acc[j];
+  }
+  if (g == 0) P[9 * kCin * kC + co] = dbacc;
+}
+
+__global__ void __launch_bounds__(256)
+sum_partials_kernel(const float* __restrict__ P, int nblk, int stride, int count,
+                    float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  float s = 0.f;
+  for (int k = 0; k < nblk; ++k) s += P[(int64_t)k * stride + i];
+  out[i] = s;
+}
+
+// ------------------------------------------------------------ optimizer --
+// Per-tensor sum of squares (one block per tensor), deterministic.
+__global__ void __launch_bounds__(256)
+sqnorm_kernel(const float* __restrict__ g, const int64_t* __restrict__ offs, float* __restrict__ sq) {
+  const int t = blockIdx.x;
+  const int64_t a = offs[t], e = offs[t + 1];
+  __shared__ float red[256];
+  float s = 0.f;
+  for (int64_t i = a + threadIdx.x; i < e; i += 256) s += g[i] * g[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sq[t] = red[0];
+}
+
+// tf.clip_by_norm per tensor, then legacy Adam on the fp32 master weights.
+__global__ void __launch_bounds__(256)
+adam_clip_kernel(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                 float* __restrict__ v, int64_t n, const int64_t* __restrict__ offs, int ntens,
+                 const float* __restrict__ sq, float clipnorm, float alpha, float omb1, float omb2,
+                 float eps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = ntens;  // offs[lo] <= i < offs[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offs[mid] <= i) lo = mid; else hi = mid;
+  }
+  float gi = g[i];
+  if (clipnorm > 0.f) {
+    const float l2sum = sq[lo];
+    const float l2norm = l2sum > 0.f ? sqrtf(l2sum) : l2sum;
+    gi = gi * clipnorm / fmaxf(l2norm, clipnorm);
+  }
+  float mi = m[i], vi = v[i];
+  mi += (gi - mi) * omb1;
+  vi += (gi * gi - vi) * omb2;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= (mi * alpha) / (sqrtf(vi) + eps);
+}
+
+// bf16 MFMA operands from the fp32 HWIO master weights of the 3x3 layers:
+// fwd[L][tap][co][ci] = W[tap][ci][co];  dgrad[L][tap][ci][co] = W[8 - tap][ci][co].
+__global__ void __launch_bounds__(256)
+pack3x3_kernel(const float* __restrict__ w, const int64_t* __restrict__ offs, int nlayers,
+               bf16* __restrict__ fwd, bf16* __restrict__ dgr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = 9 * kC * kC;
+  if (i >= (int64_t)nlayers * per) return;
+  const int L = (int)(i / per), r = (int)(i % per);
+  const int tap = r / (kC * kC), a = (r / kC) % kC, c = r % kC;
+  const float* W = w + offs[L];
+  // fwd element [tap][co=a][ci=c]
+  fwd[i] = f2bf(W[(tap * kC + c) * kC + a]);
+  // dgrad element [tap][out=ci=a][in=co=c] = W[8-tap][ci=a][co=c]
+  dgr[i] = f2bf(W[((8 - tap) * kC + a) * kC + c]);
+}
+
+// head operand [32 co (zero-padded)][32 ci] from HWIO [ci][cout].
+__global__ void pack_head_kernel(const float* __restrict__ w, int cout, bf16* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kC * kC) return;
+  const int co = i / kC, ci = i % kC;
+  out[i] = f2bf(co < cout ? w[ci * cout + co] : 0.f);
+}
+
+inline int grid_for(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+}  // namespace igm
+
+// =================================================================== C-ABI ==
+using namespace igm;
+
+extern "C" {
+
+int igm_conv_in_fwd(const float* f0, const float* f1, const float* f2, const float* f3,
+                    const float* f4, int64_t fb, int64_t fr, int64_t fc, const int32_t* extents,
+                    int32_t B, int32_t H, int32_t W, const float* w_hwio, const float* bias,
+                    void* out_bf16, void* stream) {
+  IGM_CHECK_ARG(f0 && f1 && f2 && f3 && f4 && w_hwio && bias && out_bf16, "null pointer");
+  IGM_CHECK_ARG(B >= 1 && H >= 1 && W >= 1 && H <= 65535 && B <= 65535, "bad sizes");
+  hipLaunchKernelGGL(conv_in_kernel, dim3(grid_for(W), H, B), dim3(256), 0, (hipStream_t)stream,
+                     f0, f1, f2, f3, f4, fb, fr, fc, extents, H, W, w_hwio, bias,
+                     (bf16*)out_bf16);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_conv3x3_fwd_bf16(const void* in, const void* wpacked, const float* bias,
+                         const int32_t* extents, int32_t B, int32_t H, int32_t W, void* out,
+                         void* stream) {
+  IGM_CHECK_ARG(in && wpacked && bias && out, "null pointer");
+  IGM_CHECK_ARG(in != out, "in and out must differ");
+  IGM_CHECK_ARG(B >= 1 && H >= 1 && W >= 1 && B <= 65535, "bad sizes");
+  dim3 grid((W + kTileW - 1) / kTileW, (H + kTileH - 1) / kTileH, B);
+  hipLaunchKernelGGL(conv3x3_mfma_kernel<0>, grid, dim3(256), 0, (hipStream_t)stream,
+                     (const bf16*)in, (const bf16*)wpacked, bias, (const bf16*)nullptr, extents, H, W,
+                     (bf16*)out);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_conv3x3_dgrad_bf16(const void* dz, const void* wpacked_dgrad, const void* act_below,
+                           const int32_t* extents, int32_t B, int32_t H, int32_t W, void* dz_below,
+                           void* stream) {
+  IGM_CHECK_ARG(dz && wpacked_dgrad && act_below && dz_below, "null pointer");
+  IGM_CHECK_ARG(dz != dz_below, "in and out must differ");
+  IGM_CHECK_ARG(B >= 1 && H >= 1 && W >= 1 && B <= 65535, "bad sizes");
+  dim3 grid((W + kTileW - 1) / kTileW, (H + kTileH - 1) / kTileH, B);
+  hipLaunchKernelGGL(conv3x3_mfma_kernel<1>, grid, dim3(256), 0, (hipStream_t)stream,
+                     (const bf16*)dz, (const bf16*)wpacked_dgrad, (const float*)nullptr,
+                     (const bf16*)act_below, extents, H, W, (bf16*)dz_below);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_conv1x1_head_fwd(const void* in, const void* wpacked, const float* bias,
+                         const int32_t* extents, int32_t B, int32_t H, int32_t W, int32_t cout,
+                         float* Y, void* stream) {
+  IGM_CHECK_ARG(in && wpacked && bias && Y, "null pointer");
+  IGM_CHECK_ARG(cout >= 1 && cout <= kC, "cout must be in [1, 32]");
+  IGM_CHECK_ARG(B >= 1 && H >= 1 && W >= 1 && B <= 65535, "bad sizes");
+  const int64_t npx = (int64_t)H * W;
+  dim3 grid((unsigned)((npx + 255) / 256), B);
+  hipLaunchKernelGGL(head_mfma_kernel, grid, dim3(256), 0, (hipStream_t)stream, (const bf16*)in,
+                     (const bf16*)wpacked, bias, extents, H, W, cout, Y);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_conv1x1_head_bwd(const float* dY, const void* a16, const float* w_hwio,
+                         const int32_t* extents, int32_t B, int32_t H, int32_t W, int32_t cout,
+                         void* dz16, void* dY_bf16, void* stream) {
+  IGM_CHECK_ARG(dY && a16 && w_hwio && dz16 && dY_bf16, "null pointer");
+  IGM_CHECK_ARG(cout >= 1 && cout <= kC, "cout must be in [1, 32]");
+  IGM_CHECK_ARG(B >= 1 && H >= 1 && W >= 1 && B <= 65535, "bad sizes");
+  const int64_t npx = (int64_t)H * W;
+  dim3 grid((unsigned)((npx + 255) / 256), B);
+  hipLaunchKernelGGL(head_bwd_kernel, grid, dim3(256), 0, (hipStream_t)stream, dY,
+                     (const bf16*)a16, w_hwio, extents, H, W, cout, (bf16*)dz16, (bf16*)dY_bf16);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int64_t igm_wgrad_workspace_floats(int32_t taps, int32_t nblocks) {
+  if (taps == 9) return (int64_t)nblocks * (9 * 1024 + 32);
+  if (taps == 1) return (int64_t)nblocks * (1024 + 32);
+  if (taps == 0) return (int64_t)nblocks * (9 * 5 * 32 + 32);  // layer 1
+  return 0;
+}
+
+int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int32_t* extents,
+                        int32_t B, int32_t H, int32_t W, int32_t cout, int32_t nblocks,
+                        float* workspace, float* dW_hwio, float* db, void* stream) {
+  IGM_CHECK_ARG(act && dz && workspace && dW_hwio && db, "null pointer");
+  IGM_CHECK_ARG(taps == 9 || taps == 1, "taps must be 9 or 1");
+  IGM_CHECK_ARG(cout >= 1 && cout <= kC, "cout must be in [1, 32]");
+  IGM_CHECK_ARG(nblocks >= 1 && B >= 1 && H >= 1 && W >= 1, "bad sizes");
+  hipStream_t st = (hipStream_t)stream;
+  if (taps == 9)
+    hipLaunchKernelGGL(wgrad_mfma_kernel<9>, dim3(nblocks), dim3(256), 0, st, (const bf16*)act,
+                       (const bf16*)dz, extents, B, H, W, workspace);
+  else
+    hipLaunchKernelGGL(wgrad_mfma_kernel<1>, dim3(nblocks), dim3(256), 0, st, (const bf16*)act,
+                       (const bf16*)dz, extents, B, H, W, workspace);
+  IGM_CHECK_LAUNCH();
+  const int n = taps * kC * cout + cout;
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid_for(n)), dim3(256), 0, st, workspace, nblocks,
+                     taps, cout, dW_hwio, db);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const float* f3,
+                      const float* f4, int64_t fb, int64_t fr, int64_t fc, const int32_t* extents,
+                      int32_t B, int32_t H, int32_t W, const void* dz1, int32_t nblocks,
+                      float* workspace, float* dW_hwio, float* db, void* stream) {
+  IGM_CHECK_ARG(f0 && f1 && f2 && f3 && f4 && dz1 && workspace && dW_hwio && db, "null pointer");
+  IGM_CHECK_ARG(nblocks >= 1 && B >= 1 && H >= 1 && W >= 1, "bad sizes");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(conv_in_wgrad_kernel, dim3(nblocks), dim3(256), 0, st, f0, f1, f2, f3, f4, fb,
+                     fr, fc, extents, B, H, W, (const bf16*)dz1, workspace);
+  IGM_CHECK_LAUNCH();
+  const int n = 9 * kCin * kC + kC;
+  // the layer-1 partial layout is [blk][1440 + 32] = [blk][n]: a plain column sum
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(grid_for(9 * kCin * kC)), dim3(256), 0, st,
+                     workspace, nblocks, n, 9 * kCin * kC, dW_hwio);
+  IGM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, workspace + 9 * kCin * kC,
+                     nblocks, n, kC, db);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_adam_clip_f32(float* w, const float* g, float* m, float* v, int64_t n, const int64_t* offs,
+                      int32_t ntens, float* sq_workspace, double clipnorm, const IgmAdam* adam,
+                      void* stream) {
+  IGM_CHECK_ARG(w && g && m && v && offs && sq_workspace && adam, "null pointer");
+  IGM_CHECK_ARG(ntens >= 1 && n >= 1, "bad sizes");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(sqnorm_kernel, dim3(ntens), dim3(256), 0, st, g, offs, sq_workspace);
+  IGM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(adam_clip_kernel, dim3(grid_for(n)), dim3(256), 0, st, w, g, m, v, n, offs,
+                     ntens, sq_workspace, (float)clipnorm, (float)adam->alpha,
+                     (float)(1.0 - adam->beta1), (float)(1.0 - adam->beta2), (float)adam->epsilon);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_pack_conv_weights(const float* w, const int64_t* offs3x3, int32_t nlayers3x3,
+                          const float* w_head, int32_t cout, void* fwd, void* dgrad, void* head,
+                          void* stream) {
+  IGM_CHECK_ARG(w && offs3x3 && w_head && fwd && dgrad && head, "null pointer");
+  IGM_CHECK_ARG(cout >= 1 && cout <= kC && nlayers3x3 >= 0, "bad sizes");
+  hipStream_t st = (hipStream_t)stream;
+  if (nlayers3x3 > 0) {
+    hipLaunchKernelGGL(pack3x3_kernel, dim3(grid_for((int64_t)nlayers3x3 * 9 * kC * kC)), dim3(256),
+                       0, st, w, offs3x3, nlayers3x3, (bf16*)fwd, (bf16*)dgrad);
+    IGM_CHECK_LAUNCH();
+  }
+  hipLaunchKernelGGL(pack_head_kernel, dim3(grid_for(kC * kC)), dim3(256), 0, st, w_head, cout,
+                     (bf16*)head);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // extern "C"

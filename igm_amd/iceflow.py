@@ -1,0 +1,249 @@
+"""Drop-in ``iceflow`` process module: initialize / update / finalize.
+
+Same contract as igm/processes/iceflow/iceflow.py:46-119 (validated by
+igm/common/runner/modules/validator.py:1-9): reads ``state.thk``,
+``state.usurf``, ``state.dX`` (+ ``arrhenius``, ``slidingco``, ``U``, ``V``
+when present) and rebinds ``state.U``, ``state.V`` (Nz, Ny, Nx) and
+``uvelbase, vvelbase, uvelsurf, vvelsurf, ubar, vbar`` (Ny, Nx), plus
+``vert_weight``, ``levels``, ``zeta``, ``dzeta``, ``PAD`` and the cost
+records.  Fields are torch tensors on the GPU (numpy inputs are accepted and
+moved there).  All field arithmetic runs in the HIP kernels of
+libigm_amd.so; this module is host control flow.
+
+Methods (cfg.processes.iceflow.method):
+* ``solved``     -- the plain energy solver (solve/solve.py:169-188);
+* ``emulated``   -- the CNN emulator with periodic physics-informed
+  retraining (emulate/emulate.py:116-281), plus an energy-gap fallback to
+  the solver (absent from the reference; cfg.processes.iceflow.emulator.fallback);
+* ``diagnostic`` -- emulated, plus a periodic warm-started solve that logs
+  the emulator-vs-solver misfit (diagnostic/diagnostic.py:16-84).
+"""
+import os
+
+import numpy as np
+import torch
+
+from . import config as cfgmod
+from . import emulator as emu
+from . import solver as slv
+from .energy import EnergyOperator
+from .fields import as_device, velocity_outputs
+from .vert_disc import compute_levels, compute_zeta_dzeta, define_vertical_weight
+
+FIELDIN = ("thk", "usurf", "arrhenius", "slidingco", "dX")
+
+
+def _check_supported(cfg):
+    ice = cfg.processes.iceflow
+    if ice.numerics.vert_basis != "Lagrange":
+        if ice.numerics.vert_basis in ("Legendre", "SIA"):
+            raise NotImplementedError(f"vert_basis {ice.numerics.vert_basis!r} is not supported")
+        raise ValueError(f"Unknown vertical basis: {ice.numerics.vert_basis}")  # iceflow.py:82
+    if int(ice.numerics.staggered_grid) != 1:
+        raise NotImplementedError("only staggered_grid=1 is supported")
+    if list(ice.physics.energy_components) != ["shear", "sliding_weertman", "gravity"]:
+        raise NotImplementedError("energy_components must be ['shear', 'sliding_weertman', 'gravity']")
+    if len(ice.physics.sliding_law) > 0:
+        raise NotImplementedError("sliding_law is not supported")
+    if int(ice.physics.dim_arrhenius) != 2:
+        raise NotImplementedError("dim_arrhenius must be 2")
+    if ice.physics.force_negative_gravitational_energy:
+        raise NotImplementedError("force_negative_gravitational_energy is not supported")
+    if ice.solver.lbfgs:
+        # solve.py:292-293
+        raise ValueError("solve_iceflow_lbfgs formely implemented, not working yet, will be updated.")
+    if ice.emulator.exclude_borders != 0 or ice.emulator.network.multiple_window_size != 0:
+        raise NotImplementedError("exclude_borders / multiple_window_size are not supported")
+    if ice.emulator.network.architecture != "cnn" or ice.emulator.network.cnn3d_for_vertical:
+        raise NotImplementedError("only the 'cnn' emulator architecture is supported")
+
+
+def _fields_to_device(state):
+    for f in FIELDIN + ("U", "V"):
+        if hasattr(state, f):
+            setattr(state, f, as_device(getattr(state, f)))
+
+
+def initialize_iceflow_fields(cfg, state):
+    """iceflow/utils.py:15-32."""
+    ph = cfg.processes.iceflow.physics
+    Nz = int(cfg.processes.iceflow.numerics.Nz)
+    if not hasattr(state, "arrhenius"):
+        state.arrhenius = torch.ones_like(state.thk) * (ph.init_arrhenius * ph.enhancement_factor)
+    if not hasattr(state, "slidingco"):
+        state.slidingco = torch.ones_like(state.thk) * ph.init_slidingco
+    if not hasattr(state, "U"):
+        state.U = torch.zeros((Nz,) + tuple(state.thk.shape), dtype=torch.float32, device=state.thk.device)
+        state.V = torch.zeros_like(state.U)
+
+
+def initialize_iceflow_solver(cfg, state):
+    """solve.py:132-141: one optimizer for all solves (its step counter persists)."""
+    state.optimizer = slv.LegacyAdamState(lr=float(cfg.processes.iceflow.solver.step_size))
+
+
+def energy_operator(cfg, state, dtype=torch.float32):
+    fields = {f: getattr(state, f) for f in FIELDIN}
+    if dtype != torch.float32:
+        fields = {k: v.to(dtype) for k, v in fields.items()}
+    return EnergyOperator(np.asarray(state.dzeta), cfg.processes.iceflow.physics, fields=fields,
+                          dtype=dtype)
+
+
+def solve_iceflow(cfg, state, U, V, nbitmax=None):
+    """solve.py:143-240 on the GPU.  Returns (U, V) unmasked and the cost list."""
+    ice = cfg.processes.iceflow
+    op = energy_operator(cfg, state)
+    res = slv.solve(op, U, V, int(ice.solver.nbitmax if nbitmax is None else nbitmax), state.optimizer,
+                    early_stopping=True, check_every=int(cfgmod.get(ice, "solver.check_every", 16)))
+    return res
+
+
+def _set_outputs(cfg, state, out):
+    for k, v in out.items():
+        setattr(state, k, v)
+
+
+def update_iceflow_solved(cfg, state):
+    """solve.py:290-309."""
+    ice = cfg.processes.iceflow
+    res = solve_iceflow(cfg, state, state.U, state.V)
+    out = velocity_outputs(res.U, res.V, state.thk, state.vert_weight, int(ice.numerics.Nz))
+    _set_outputs(cfg, state, out)
+    if ice.force_max_velbar > 0:
+        raise NotImplementedError("force_max_velbar is not supported")
+    if len(ice.solver.save_cost) > 0:
+        np.savetxt(ice.emulator.output_directory + ice.solver.save_cost + "-" + str(state.it) + ".dat",
+                   np.asarray(res.costs), fmt="%5.10f")
+    state.COST_Glen = float(res.costs[-1]) if len(res.costs) else float("nan")
+    state.Cost_Glen = res.costs
+    state.solver_iterations = res.iterations
+
+
+def initialize(cfg, state):
+    """iceflow.py:46-95."""
+    if hasattr(state, "was_initialize_iceflow_already_called"):
+        return
+    _check_supported(cfg)
+    _fields_to_device(state)
+    initialize_iceflow_fields(cfg, state)
+    ice = cfg.processes.iceflow
+    if ice.method == "emulated":
+        emu.initialize_iceflow_emulator(cfg, state)
+    elif ice.method == "solved":
+        initialize_iceflow_solver(cfg, state)
+    elif ice.method == "diagnostic":
+        emu.initialize_iceflow_emulator(cfg, state)
+        initialize_iceflow_solver(cfg, state)
+        state.diagno = []
+        state.tlast_diagno = -(10 ** 10)
+    else:
+        raise ValueError(f"Unknown iceflow method: {ice.method}")
+    if ice.method == "emulated" and cfgmod.get(ice, "emulator.fallback.enabled", False):
+        initialize_iceflow_solver(cfg, state)
+
+    Nz, vs = int(ice.numerics.Nz), float(ice.numerics.vert_spacing)
+    state.vert_weight = define_vertical_weight(Nz, vs)
+    state.levels = compute_levels(Nz, vs)
+    state.zeta, state.dzeta = compute_zeta_dzeta(state.levels)
+    state.Leg_P, state.Leg_dPdz, state.Leg_I = None, None, None
+    state.PAD = [[0, 0], [0, 0], [0, 0], [0, 0]]
+
+    if not ice.method == "solved":
+        emu.update_iceflow_emulator(cfg, state, 0)
+        emu.update_iceflow_emulated(cfg, state)
+    state.was_initialize_iceflow_already_called = True
+
+
+def update(cfg, state):
+    """iceflow.py:97-113 (+ the energy-gap fallback for 'emulated')."""
+    if hasattr(state, "logger"):
+        state.logger.info("Update ICEFLOW at iteration : " + str(state.it))
+    _fields_to_device(state)
+    ice = cfg.processes.iceflow
+    if ice.method in ("emulated", "diagnostic"):
+        if (ice.emulator.retrain_freq > 0) & (state.it > 0):
+            emu.update_iceflow_emulator(cfg, state, state.it, pertubate=ice.emulator.pertubate)
+        emu.update_iceflow_emulated(cfg, state)
+        if ice.method == "emulated" and cfgmod.get(ice, "emulator.fallback.enabled", False):
+            fallback_check(cfg, state)
+        if ice.method == "diagnostic":
+            update_iceflow_diagnostic(cfg, state)
+    elif ice.method == "solved":
+        update_iceflow_solved(cfg, state)
+
+
+def finalize(cfg, state):
+    """iceflow.py:116-119."""
+    if cfg.processes.iceflow.emulator.save_model:
+        emu.save_iceflow_model(cfg, state)
+
+
+# ----------------------------------------------------------- fallback --
+
+def fallback_check(cfg, state):
+    """Energy-gap fallback of the emulated field to the plain solver.
+
+    Every ``check_freq`` calls: J_emu = J(emulated U, V) (float64 reduction).
+    J_check is the lowest COST seen by ``check_its`` solver iterations started
+    from the emulated field.  If J_emu > J_check + rel_energy_gap * |J_check|,
+    the solve continues (up to ``max_its`` iterations with early stopping) and
+    its field replaces the emulated one.  With check_its = 0 the check only
+    records J_emu.  Records state.COST_EMULATED, state.COST_CHECK and
+    state.fallback_count.
+    """
+    fb = cfg.processes.iceflow.emulator.fallback
+    state.fallback_calls = getattr(state, "fallback_calls", 0) + 1
+    state.fallback_count = getattr(state, "fallback_count", 0)
+    if (state.fallback_calls - 1) % max(1, int(fb.check_freq)) != 0:
+        return
+    op = energy_operator(cfg, state)
+    j_emu = float(op.energy(state.U, state.V)[0, 3])
+    state.COST_EMULATED = j_emu
+    k = int(fb.check_its)
+    if k <= 0:
+        return
+    res = slv.solve(op, state.U, state.V, k, state.optimizer, early_stopping=False)
+    j_chk = float(min(np.min(res.costs), j_emu))
+    state.COST_CHECK = j_chk
+    if j_emu > j_chk + float(fb.rel_energy_gap) * abs(j_chk):
+        state.fallback_count += 1
+        res2 = slv.solve(op, res.U, res.V, int(fb.max_its), state.optimizer, early_stopping=True)
+        out = velocity_outputs(res2.U, res2.V, state.thk, state.vert_weight,
+                               int(cfg.processes.iceflow.numerics.Nz))
+        _set_outputs(cfg, state, out)
+        state.COST_Glen = float(res2.costs[-1]) if len(res2.costs) else j_chk
+
+
+# --------------------------------------------------------- diagnostic --
+
+def update_iceflow_diagnostic(cfg, state):
+    """diagnostic/diagnostic.py:25-71: every 10 years, solve warm-started
+    from the emulated field and log the misfit and both costs."""
+    t = float(getattr(state, "t", 0.0))
+    if (t - state.tlast_diagno) < 10:
+        return
+    res = solve_iceflow(cfg, state, state.U, state.V)
+    Nz = int(cfg.processes.iceflow.numerics.Nz)
+    sol = velocity_outputs(res.U, res.V, state.thk, state.vert_weight, Nz)
+    l1, l2 = computemisfit(state.thk, state.ubar - sol["ubar"], state.vbar - sol["vbar"])
+    vol = float(torch.sum(state.thk, dtype=torch.float64)) * float(state.dX[0, 0]) ** 2 / 1e9
+    cost_emu = float(state.COST_EMULATOR[-1]) if getattr(state, "COST_EMULATOR", None) else float("nan")
+    state.diagno.append([t, l1, l2, float(res.costs[-1]), cost_emu, res.iterations,
+                         len(getattr(state, "COST_EMULATOR", [])), vol])
+    np.savetxt("errors_diagno.txt", np.stack(state.diagno), delimiter=",", fmt="%10.3f",
+               header="time,l1,l2,COST_Glen,COST_Emulator,nb_it_solve,nb_it_emul,vol", comments="")
+    state.tlast_diagno = t
+
+
+def computemisfit(thk, dubar, dvbar):
+    """diagnostic.py:74-84 on the velbar difference (masked thk > 1)."""
+    m = (thk > 1).to(torch.float64)
+    n = 2.0 * float(torch.sum(m))
+    if n == 0:
+        return 0.0, 0.0
+    a = torch.abs(dubar.double()) * m
+    b = torch.abs(dvbar.double()) * m
+    l1 = float(torch.sum(a) + torch.sum(b)) / n
+    l2 = float(torch.sum(a * a) + torch.sum(b * b)) / n
+    return l1, float(np.sqrt(l2))

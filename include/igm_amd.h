@@ -144,6 +144,81 @@ int igm_velocity_outputs_f64(int32_t nz, int32_t Ny, int32_t Nx, const double* U
                              double* uvelbase, double* vvelbase, double* uvelsurf,
                              double* vvelsurf, double* ubar, double* vbar, void* stream);
 
+/* ------------------------------------------------------------ emulator -- */
+/* The "cnn" emulator (emulate/neural_network.py:9-66): nb_layers x
+ * [Conv2D 3x3 "same" + bias -> LeakyReLU(0.01)] -> Conv2D 1x1 + bias, with
+ * 5 input fields and 32 filters.  Activations: NHWC bf16 (B, H, W, 32);
+ * output Y: NHWC fp32 (B, H, W, cout).  `extents` (device, nullable): per
+ * batch entry (ny, nx); outputs outside are written as zeros, so a padded
+ * batch equals per-entry "same" zero padding.  Weights are Keras HWIO fp32
+ * masters; MFMA layers take packed bf16 operands (igm_pack_conv_weights). */
+
+/* Layer 1 (Conv2D 3x3, 5 -> 32, neural_network.py:23-34), fp32 arithmetic on
+ * the raw fields [thk, usurf, arrhenius, slidingco, dX] (iceflow/utils.py:147-158)
+ * read with element strides (fb, fr, fc); bias + LeakyReLU; bf16 output. */
+int igm_conv_in_fwd(const float* f0, const float* f1, const float* f2, const float* f3,
+                    const float* f4, int64_t fb, int64_t fr, int64_t fc, const int32_t* extents,
+                    int32_t B, int32_t H, int32_t W, const float* w_hwio, const float* bias,
+                    void* out_bf16, void* stream);
+
+/* Layers 2..nb_layers (Conv2D 3x3, 32 -> 32) + bias + LeakyReLU on bf16 MFMA
+ * (fp32 accumulate).  wpacked: [9][32 out][32 in] bf16. */
+int igm_conv3x3_fwd_bf16(const void* in, const void* wpacked, const float* bias,
+                         const int32_t* extents, int32_t B, int32_t H, int32_t W, void* out,
+                         void* stream);
+
+/* Input gradient of a 3x3 layer (the tape through Conv2D, emulate.py:252):
+ * dz_below = conv(dz, flipped W) * LeakyReLU'(act_below).
+ * wpacked_dgrad: [9][32 in][32 out] bf16 with the taps reversed. */
+int igm_conv3x3_dgrad_bf16(const void* dz, const void* wpacked_dgrad, const void* act_below,
+                           const int32_t* extents, int32_t B, int32_t H, int32_t W, void* dz_below,
+                           void* stream);
+
+/* Output layer Conv2D 1x1, 32 -> cout (<= 32), no activation
+ * (neural_network.py:56-64): Y = A W + b, fp32.  wpacked: [32][32] bf16. */
+int igm_conv1x1_head_fwd(const void* in, const void* wpacked, const float* bias,
+                         const int32_t* extents, int32_t B, int32_t H, int32_t W, int32_t cout,
+                         float* Y, void* stream);
+
+/* Backward of the output layer: dz16 = (dY W^T) * LeakyReLU'(a16) (bf16) and
+ * dY_bf16 = bf16(dY) zero-padded to 32 channels (input of its wgrad).
+ * w_hwio: [32][cout] fp32. */
+int igm_conv1x1_head_bwd(const float* dY, const void* a16, const float* w_hwio,
+                         const int32_t* extents, int32_t B, int32_t H, int32_t W, int32_t cout,
+                         void* dz16, void* dY_bf16, void* stream);
+
+/* Floats of workspace for igm_conv_wgrad_bf16 (taps 9 or 1) or
+ * igm_conv_in_wgrad (taps 0) with `nblocks` partial-sum blocks. */
+int64_t igm_wgrad_workspace_floats(int32_t taps, int32_t nblocks);
+
+/* Weight and bias gradient of a 3x3 (taps=9) or 1x1 (taps=1) layer on bf16
+ * MFMA: dW_hwio [taps][32][cout] and db [cout] (fp32), summed over all
+ * pixels of the batch in a fixed order (per-block partials + reduction). */
+int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int32_t* extents,
+                        int32_t B, int32_t H, int32_t W, int32_t cout, int32_t nblocks,
+                        float* workspace, float* dW_hwio, float* db, void* stream);
+
+/* Weight and bias gradient of layer 1 (fp32): dW_hwio [3][3][5][32], db [32]. */
+int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const float* f3,
+                      const float* f4, int64_t fb, int64_t fr, int64_t fc, const int32_t* extents,
+                      int32_t B, int32_t H, int32_t W, const void* dz1, int32_t nblocks,
+                      float* workspace, float* dW_hwio, float* db, void* stream);
+
+/* Legacy Keras Adam with per-tensor clipnorm on a flat fp32 parameter vector
+ * (emulate.py:24-35, 267-269): tensor t spans [offs[t], offs[t+1]);
+ * g_t <- g_t * clipnorm / max(||g_t||, clipnorm) (tf.clip_by_norm; clipnorm
+ * <= 0 disables it), then the Adam step of IgmAdam.  sq_workspace: ntens floats. */
+int igm_adam_clip_f32(float* w, const float* g, float* m, float* v, int64_t n, const int64_t* offs,
+                      int32_t ntens, float* sq_workspace, double clipnorm, const IgmAdam* adam,
+                      void* stream);
+
+/* bf16 MFMA operands from the fp32 HWIO masters: for each 3x3 layer at
+ * w + offs3x3[L] (device int64 array), fwd [L][9][32][32] and dgrad
+ * [L][9][32][32]; head [32][32] from w_head [32][cout]. */
+int igm_pack_conv_weights(const float* w, const int64_t* offs3x3, int32_t nlayers3x3,
+                          const float* w_head, int32_t cout, void* fwd, void* dgrad, void* head,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,240 @@
+"""GPU parity of the emulator's MFMA convolution kernels.
+
+Each kernel is compared with torch-CPU float32 (or float64) on the SAME
+bf16-representable operands, so only the accumulation order and the final
+bf16 rounding of the output differ.  Tolerances (rel L2): 4e-3 for bf16
+outputs (half an ulp of bf16 is 2^-9 = 2e-3), 1e-5 for fp32 outputs.
+Whole-network checks compare with the oracle CNN (oracle/cnn.py) run with
+bf16 rounding of weights and activations at the same places.
+"""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from igm_amd import _lib as L
+from igm_amd import emulator as EM
+from oracle import cnn as OC
+from tests.helpers import rel, small_glacier
+
+pytestmark = pytest.mark.gpu
+
+
+def bf(x):
+    return torch.as_tensor(x, dtype=torch.float32).to(torch.bfloat16).to(torch.float32)
+
+
+def nchw(t):
+    return t.permute(0, 3, 1, 2)
+
+
+def nhwc(t):
+    return t.permute(0, 2, 3, 1)
+
+
+@pytest.mark.parametrize("B,H,W", [(1, 7, 5), (2, 33, 70), (1, 64, 64), (3, 17, 130)])
+def test_conv3x3_fwd_and_dgrad(dev, B, H, W):
+    print(f"case conv3x3 B={B} H={H} W={W}", flush=True)
+    g = torch.Generator().manual_seed(H * W)
+    x = bf(torch.randn((B, H, W, 32), generator=g))
+    w = bf(torch.randn((3, 3, 32, 32), generator=g) * 0.1)  # HWIO
+    b = torch.randn(32, generator=g) * 0.1
+    cnn = EM.CNN(nz=10, nb_layers=2, device=dev)
+    wts = cnn.get_weights()
+    wts[1] = (w.numpy(), b.numpy())
+    cnn.set_weights(wts)
+    xd = x.to(torch.bfloat16).to(dev)
+    out = torch.empty_like(xd)
+    L.call("igm_conv3x3_fwd_bf16", L.ptr(xd), L.ptr(cnn.fwd[0]), L.ptr(cnn.bias(1)), None, B, H, W,
+           L.ptr(out), L.stream_ptr())
+    ref = F.leaky_relu(F.conv2d(nchw(x), w.permute(3, 2, 0, 1), b, padding=1), 0.01)
+    assert rel(out.float().cpu(), nhwc(ref)) <= 4e-3
+    # dgrad: dz_below = conv_transpose(dz) * leaky'(act_below)
+    dz = bf(torch.randn((B, H, W, 32), generator=g))
+    act_below = bf(torch.randn((B, H, W, 32), generator=g))
+    dzd = dz.to(torch.bfloat16).to(dev)
+    abd = act_below.to(torch.bfloat16).to(dev)
+    outd = torch.empty_like(dzd)
+    L.call("igm_conv3x3_dgrad_bf16", L.ptr(dzd), L.ptr(cnn.dgr[0]), L.ptr(abd), None, B, H, W, L.ptr(outd),
+           L.stream_ptr())
+    xi = nchw(act_below).clone().requires_grad_(True)
+    y = F.conv2d(xi, w.permute(3, 2, 0, 1), None, padding=1)
+    (gx,) = torch.autograd.grad(y, xi, nchw(dz))
+    refd = nhwc(gx) * torch.where(act_below > 0, 1.0, 0.01)
+    assert rel(outd.float().cpu(), refd) <= 4e-3
+
+
+@pytest.mark.parametrize("B,H,W", [(1, 9, 11), (2, 40, 100)])
+def test_wgrad_3x3_and_1x1(dev, B, H, W):
+    g = torch.Generator().manual_seed(7 + H)
+    a = bf(torch.randn((B, H, W, 32), generator=g))
+    dz = bf(torch.randn((B, H, W, 32), generator=g))
+    ws = torch.empty(L.load().igm_wgrad_workspace_floats(9, 64), dtype=torch.float32, device=dev)
+    dW = torch.empty((3, 3, 32, 32), dtype=torch.float32, device=dev)
+    db = torch.empty(32, dtype=torch.float32, device=dev)
+    L.call("igm_conv_wgrad_bf16", 9, L.ptr(a.to(torch.bfloat16).to(dev)), L.ptr(dz.to(torch.bfloat16).to(dev)),
+           None, B, H, W, 32, 64, L.ptr(ws), L.ptr(dW), L.ptr(db), L.stream_ptr())
+    wt = torch.zeros((32, 32, 3, 3), dtype=torch.float64, requires_grad=True)
+    y = F.conv2d(nchw(a).double(), wt, None, padding=1)
+    (gw,) = torch.autograd.grad(y, wt, nchw(dz).double())
+    assert rel(dW.cpu(), gw.permute(2, 3, 1, 0)) <= 1e-5
+    assert rel(db.cpu(), dz.double().sum(dim=(0, 1, 2))) <= 1e-5
+    # 1x1 with cout = 20 (zero-padded bf16 dY)
+    dY = torch.zeros_like(dz)
+    dY[..., :20] = dz[..., :20]
+    dW1 = torch.empty((1, 1, 32, 20), dtype=torch.float32, device=dev)
+    db1 = torch.empty(20, dtype=torch.float32, device=dev)
+    L.call("igm_conv_wgrad_bf16", 1, L.ptr(a.to(torch.bfloat16).to(dev)), L.ptr(dY.to(torch.bfloat16).to(dev)),
+           None, B, H, W, 20, 64, L.ptr(ws), L.ptr(dW1), L.ptr(db1), L.stream_ptr())
+    ref = torch.einsum("bhwi,bhwo->io", a.double(), dY[..., :20].double())
+    assert rel(dW1.cpu()[0, 0], ref) <= 1e-5
+    assert rel(db1.cpu(), dY[..., :20].double().sum(dim=(0, 1, 2))) <= 1e-5
+
+
+def test_conv_in_and_head(dev):
+    B, H, W = 2, 37, 90
+    gl = [small_glacier(H, W, seed=s) for s in range(B)]
+    X = torch.stack([torch.stack([torch.as_tensor(gg[k]) for k in ("thk", "usurf", "arrhenius", "slidingco", "dX")], -1)
+                     for gg in gl]).float()
+    cnn = EM.CNN(nz=10, nb_layers=1, device=dev, seed=3)
+    k0, b0 = cnn.get_weights()[0]
+    Xd = X.to(dev).contiguous()
+    fields = EM.nhwc_fields(Xd)
+    out = torch.empty((B, H, W, 32), dtype=torch.bfloat16, device=dev)
+    L.call("igm_conv_in_fwd", *fields[0], *fields[1], None, B, H, W, L.ptr(cnn.kernel(0)), L.ptr(cnn.bias(0)),
+           L.ptr(out), L.stream_ptr())
+    ref = nhwc(F.leaky_relu(F.conv2d(nchw(X).double(), torch.as_tensor(k0).double().permute(3, 2, 0, 1),
+                                     torch.as_tensor(b0).double(), padding=1), 0.01))
+    assert rel(out.float().cpu(), ref) <= 4e-3
+    # head: 1x1 32 -> 20, fp32 out
+    a = bf(torch.randn((B, H, W, 32)))
+    Y = torch.empty((B, H, W, 20), dtype=torch.float32, device=dev)
+    L.call("igm_conv1x1_head_fwd", L.ptr(a.to(torch.bfloat16).to(dev)), L.ptr(cnn.head), L.ptr(cnn.bias(1)),
+           None, B, H, W, 20, L.ptr(Y), L.stream_ptr())
+    kh = bf(cnn.kernel(1).cpu())[0, 0]
+    refY = a.double() @ kh.double() + cnn.bias(1).cpu().double()
+    assert rel(Y.cpu(), refY) <= 1e-5
+
+
+def test_conv_in_wgrad(dev):
+    B, H, W = 2, 30, 300
+    g = torch.Generator().manual_seed(5)
+    X = torch.randn((B, H, W, 5), generator=g)
+    dz = bf(torch.randn((B, H, W, 32), generator=g))
+    ws = torch.empty(L.load().igm_wgrad_workspace_floats(0, 64), dtype=torch.float32, device=dev)
+    dW = torch.empty((3, 3, 5, 32), dtype=torch.float32, device=dev)
+    db = torch.empty(32, dtype=torch.float32, device=dev)
+    Xd = X.to(dev).contiguous()
+    fields = EM.nhwc_fields(Xd)
+    L.call("igm_conv_in_wgrad", *fields[0], *fields[1], None, B, H, W, L.ptr(dz.to(torch.bfloat16).to(dev)),
+           64, L.ptr(ws), L.ptr(dW), L.ptr(db), L.stream_ptr())
+    wt = torch.zeros((32, 5, 3, 3), dtype=torch.float64, requires_grad=True)
+    y = F.conv2d(nchw(X).double(), wt, None, padding=1)
+    (gw,) = torch.autograd.grad(y, wt, nchw(dz).double())
+    assert rel(dW.cpu(), gw.permute(2, 3, 1, 0)) <= 1e-5
+    assert rel(db.cpu(), dz.double().sum(dim=(0, 1, 2))) <= 1e-5
+
+
+def oracle_forward_bf16(X, weights):
+    """Oracle CNN with the build's rounding points: bf16 weights for layers
+    >= 2 and the head, bf16 activations after every hidden layer."""
+    h = nchw(torch.as_tensor(X, dtype=torch.float64))
+    for li, (k, b) in enumerate(weights):
+        kt = torch.as_tensor(k, dtype=torch.float64)
+        if li > 0:
+            kt = bf(kt).double()
+        h = F.conv2d(h, kt.permute(3, 2, 0, 1), torch.as_tensor(b, dtype=torch.float64), padding=kt.shape[0] // 2)
+        if li < len(weights) - 1:
+            h = bf(F.leaky_relu(h, 0.01)).double()
+    return nhwc(h)
+
+
+def test_network_forward_pretrained(dev):
+    """IGM's shipped emulator weights on a glacier: full 17-layer forward."""
+    weights, _ = EM.load_weights(EM.os.path.join(EM.EMULATOR_DIR, "pinnbp_10_4_cnn_16_32_2_1_a.safetensors"))
+    cnn = EM.CNN(nz=10, nb_layers=16, device=dev, weights=weights)
+    g = small_glacier(48, 80, dx=100.0, seed=1)
+    X = torch.stack([torch.as_tensor(g[k]) for k in ("thk", "usurf", "arrhenius", "slidingco", "dX")], -1)[None].float()
+    Y = cnn.forward(EM.nhwc_fields(X.to(dev).contiguous()))
+    ref = oracle_forward_bf16(X, weights)
+    assert rel(Y.cpu(), ref) <= 1e-2
+    # and the plain fp32 oracle (what Keras computes) within the bf16 budget
+    ref32 = OC.forward(X, weights, dtype=torch.float64)
+    print("pretrained net: rel L2 vs fp64 oracle", rel(Y.cpu(), ref32))
+    assert rel(Y.cpu(), ref32) <= 5e-2
+
+
+def test_network_backward_vs_autograd(dev):
+    """Parameter gradient of sum(dY * Y) vs torch autograd of the oracle CNN
+    (float64, bf16 rounding simulated with a straight-through estimator)."""
+    nl = 4
+    cnn = EM.CNN(nz=10, nb_layers=nl, device=dev, seed=11)
+    weights = cnn.get_weights()
+    g = small_glacier(24, 70, dx=50.0, seed=4)
+    X = torch.stack([torch.as_tensor(g[k]) for k in ("thk", "usurf", "arrhenius", "slidingco", "dX")], -1)[None].float()
+    Xd = X.to(dev).contiguous()
+    fields = EM.nhwc_fields(Xd)
+    Y, acts = cnn.forward(fields, keep=True)
+    dY = torch.randn(Y.shape, generator=torch.Generator().manual_seed(2)).float()
+    cnn.backward(dY.to(dev), acts, fields)
+    grads = [(cnn.kernel(i, cnn.g).cpu(), cnn.bias(i, cnn.g).cpu()) for i in range(nl + 1)]
+
+    def ste(x):  # bf16 rounding forward, identity backward
+        return x + (bf(x.detach()).double() - x.detach())
+
+    ps = []
+    h = nchw(X.double())
+    for li, (k, b) in enumerate(weights):
+        kt = torch.as_tensor(k, dtype=torch.float64).requires_grad_(True)
+        bt = torch.as_tensor(b, dtype=torch.float64).requires_grad_(True)
+        ps += [kt, bt]
+        kk = ste(kt) if li > 0 else kt
+        h = F.conv2d(h, kk.permute(3, 2, 0, 1), bt, padding=kt.shape[0] // 2)
+        if li < nl:
+            h = ste(F.leaky_relu(h, 0.01))
+    loss = (nhwc(h) * dY.double()).sum()
+    gr = torch.autograd.grad(loss, ps)
+    for li in range(nl + 1):
+        ek, eb = rel(grads[li][0], gr[2 * li]), rel(grads[li][1], gr[2 * li + 1])
+        print(f"layer {li}: kernel grad rel {ek:.2e}, bias grad rel {eb:.2e}")
+        assert ek <= 3e-2 and eb <= 3e-2
+
+
+def test_padded_batch_equals_single(dev):
+    cnn = EM.CNN(nz=10, nb_layers=3, device=dev, seed=5)
+    sizes = [(20, 33), (40, 64), (9, 100)]
+    H, W = 40, 100
+    X = torch.zeros((3, H, W, 5))
+    singles = []
+    for b, (ny, nx) in enumerate(sizes):
+        g = small_glacier(ny, nx, seed=b)
+        xb = torch.stack([torch.as_tensor(g[k]) for k in ("thk", "usurf", "arrhenius", "slidingco", "dX")], -1).float()
+        X[b, :ny, :nx] = xb
+        singles.append(xb)
+    X[0, 25:, :, :] = 7.0  # garbage outside the extents must not leak in
+    ext = torch.tensor(sizes, dtype=torch.int32, device=dev)
+    Y = cnn.forward(EM.nhwc_fields(X.to(dev).contiguous()), extents=ext).cpu()
+    for b, (ny, nx) in enumerate(sizes):
+        Yb = cnn.forward(EM.nhwc_fields(singles[b][None].to(dev).contiguous())).cpu()
+        assert torch.equal(Y[b, :ny, :nx], Yb[0])
+        assert float(Y[b, ny:].abs().sum()) == 0.0 and float(Y[b, :, nx:].abs().sum()) == 0.0
+
+
+def test_adam_clip(dev):
+    cnn = EM.CNN(nz=10, nb_layers=2, device=dev, seed=1)
+    w0 = cnn.w.cpu().double().clone()
+    g = torch.randn(cnn.n, generator=torch.Generator().manual_seed(3)).float()
+    cnn.g.copy_(g.to(dev))
+    cnn.adam_step(lr=1e-3, clipnorm=1.0)
+    # oracle: per-tensor clip_by_norm then legacy Adam at t = 1
+    w = w0.clone()
+    for i in range(len(cnn.shapes)):
+        a, e = cnn.offs_host[i], cnn.offs_host[i + 1]
+        gi = g[a:e].double()
+        n = float(torch.linalg.vector_norm(gi))
+        gi = gi * 1.0 / max(n, 1.0)
+        m = 0.1 * gi
+        v = 0.001 * gi * gi
+        alpha = 1e-3 * np.sqrt(1 - 0.999) / (1 - 0.9)
+        w[a:e] -= (m * alpha) / (torch.sqrt(v) + 1e-7)
+    assert rel(cnn.w.cpu(), w) <= 1e-6
