@@ -729,7 +729,8 @@ int igm_adam_clip_f32(float* w, const float* g, float* m, float* v, int64_t n, c
 int igm_pack_conv_weights(const float* w, const int64_t* offs3x3, int32_t nlayers3x3,
                           const float* w_head, int32_t cout, void* fwd, void* dgrad, void* head,
                           void* stream) {
-  IGM_CHECK_ARG(w && offs3x3 && w_head && fwd && dgrad && head, "null pointer");
+  IGM_CHECK_ARG(w && w_head && fwd && dgrad && head, "null pointer");
+  IGM_CHECK_ARG(nlayers3x3 == 0 || offs3x3, "null offs3x3");
   IGM_CHECK_ARG(cout >= 1 && cout <= kC && nlayers3x3 >= 0, "bad sizes");
   hipStream_t st = (hipStream_t)stream;
   if (nlayers3x3 > 0) {

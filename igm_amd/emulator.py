@@ -61,7 +61,9 @@ class CNN:
         self.v = torch.zeros_like(self.w)
         self.g = torch.zeros_like(self.w)
         self.offs = torch.as_tensor(self.offs_host, device=dev)
-        self.offs3x3 = torch.as_tensor(self.offs_host[2:2 * nb_layers:2], device=dev)  # kernels 1..nl-1
+        # offsets of the 3x3 MFMA kernels (layers 1..nl-1); None when there are none
+        self.offs3x3 = (torch.as_tensor(self.offs_host[2:2 * nb_layers:2], device=dev)
+                        if nb_layers > 1 else None)
         self.sq = torch.zeros(len(self.shapes), dtype=torch.float32, device=dev)
         self.fwd = torch.empty((max(1, nb_layers - 1), 9, NF, NF), dtype=torch.bfloat16, device=dev)
         self.dgr = torch.empty_like(self.fwd)
@@ -197,20 +199,33 @@ def load_weights(path):
     return weights, meta
 
 
+class Fields(tuple):
+    """(pointers of the 5 input fields, (fb, fr, fc) strides, B, H, W) that
+    also holds references to the tensors, so they outlive every kernel that
+    reads them through the raw pointers."""
+
+    def __new__(cls, keep, ptrs, strides, B, H, W):
+        obj = super().__new__(cls, (ptrs, strides, B, H, W))
+        obj.keep = keep
+        return obj
+
+
 def planar_fields(tensors):
-    """Forward/backward field descriptor for 5 planar (Ny, Nx) or (B, Ny, Nx) tensors."""
+    """Descriptor for 5 planar (Ny, Nx) or (B, Ny, Nx) fp32 device tensors."""
     ts = [t if t.dim() == 3 else t[None] for t in tensors]
-    ts = [t.contiguous() for t in ts]
+    ts = [t.to(torch.float32).contiguous() for t in ts]
+    assert all(t.shape == ts[0].shape and t.stride() == ts[0].stride() for t in ts)
     B, H, W = ts[0].shape
-    return [t.data_ptr() for t in ts], tuple(ts[0].stride()), B, H, W
+    return Fields(ts, [t.data_ptr() for t in ts], tuple(ts[0].stride()), B, H, W)
 
 
 def nhwc_fields(X):
     """Descriptor for X (B, H, W, 5) fp32 contiguous."""
-    assert X.dim() == 4 and X.shape[-1] == NIN and X.is_contiguous()
+    assert X.dim() == 4 and X.shape[-1] == NIN and X.is_contiguous() and X.dtype == torch.float32
     es = X.element_size()
     B, H, W, _ = X.shape
-    return [X.data_ptr() + f * es for f in range(NIN)], (X.stride(0), X.stride(1), X.stride(2)), B, H, W
+    return Fields([X], [X.data_ptr() + f * es for f in range(NIN)], (X.stride(0), X.stride(1), X.stride(2)),
+                  B, H, W)
 
 
 # ------------------------------------------------------------------------
