@@ -1,0 +1,319 @@
+#!/usr/bin/env python3
+"""Velocity fields/s at 1000 x 1000 x Nz=10: emulator vs energy solver at <= 1 % error.
+
+One step = one velocity field of the emulated ice-flow update on the
+synthetic 20-dome glacier (1000 x 1000, dx = 50 m, Nz = 10): CNN inference on
+the MFMA kernels, thk > 0 mask + derived fields, and the energy-gap check of
+the fallback (float64-reduced energy; fallback solve when the gap exceeds
+1 %).  Inputs are resident in HBM before the timed region.
+
+Reported next to it:
+* the float32 energy solver (the reference's plain method: legacy Adam,
+  lr = 1, cold start) timed to <= 1 % rel L2 error vs the float64 minimiser;
+* the float64 minimiser itself (float64 GPU solve, decaying step);
+* live rooflines (HIP events on the launch stream) of the dominant kernel
+  (the 3x3 bf16 MFMA conv) and of the energy/adjoint kernel;
+* a CPU baseline: the oracle (torch-CPU float32 restatement) CNN inference
+  and solver iteration on a bounded sample, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "velocity fields/s at 1000²×Nz10: emulator vs energy solver at ≤1% rel err"
+PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0
+BENCH_WEIGHTS = ("igm_amd_cnn_16_32_nz10.safetensors", "pinnbp_10_4_cnn_16_32_2_1_a.safetensors")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_err(U, V, Ur, Vr, mask):
+    d = torch.stack([U, V]).double() - torch.stack([Ur, Vr]).double()
+    r = torch.stack([Ur, Vr]).double()
+    m = mask.to(torch.float64)
+    return float(torch.linalg.vector_norm(d * m) / torch.linalg.vector_norm(r * m))
+
+
+class Timer:
+    """HIP events on torch's current stream (the stream every kernel here is launched on)."""
+
+    def __init__(self):
+        self.a = torch.cuda.Event(enable_timing=True)
+        self.b = torch.cuda.Event(enable_timing=True)
+
+    def __enter__(self):
+        self.a.record()
+        return self
+
+    def __exit__(self, *exc):
+        self.b.record()
+        self.b.synchronize()
+        self.ms = self.a.elapsed_time(self.b)
+
+
+def reference_solution(f64, thk, nz, dzeta, its_flat, its_decay):
+    """float64 minimiser: legacy Adam lr = 1, then a geometric decay of the step."""
+    from igm_amd.energy import EnergyOperator
+    from igm_amd.solver import LegacyAdamState, solve
+    from oracle.energy import DEFAULT_PHYSICS
+    op = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f64, dtype=torch.float64)
+    z = torch.zeros((nz,) + tuple(thk.shape), dtype=torch.float64, device=thk.device)
+    opt = LegacyAdamState(lr=1.0)
+    decay = (1e-3) ** (1.0 / max(1, its_decay))
+    sched = lambda i: 1.0 if i < its_flat else max(1e-3, decay ** (i - its_flat))
+    r = solve(op, z, z.clone(), its_flat + its_decay, opt, early_stopping=False, lr_schedule=sched)
+    # convergence indicator: change over a further float64 refinement
+    r2 = solve(op, r.U, r.V, max(100, its_decay // 10), opt, early_stopping=False, lr_schedule=lambda i: 1e-3)
+    change = rel_err(r.U, r.V, r2.U, r2.V, thk > 0)
+    return r2.U, r2.V, float(r2.costs[-1]), change, op
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--ref-flat", type=int, default=3000)
+    ap.add_argument("--ref-decay", type=int, default=6000)
+    ap.add_argument("--solver-max", type=int, default=20000)
+    ap.add_argument("--check-its", type=int, default=None, help="fallback check iterations (cfg default if unset)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--weights", default=None)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the ice-flow path runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
+        sys.exit(2)
+
+    from igm_amd import config, emulator as EM, iceflow, synthetic as syn
+    from igm_amd import _lib as L
+    from igm_amd.energy import EnergyOperator
+    from igm_amd.fields import velocity_outputs
+    from igm_amd.solver import LegacyAdamState, solve
+    from oracle.energy import DEFAULT_PHYSICS
+
+    dev = torch.device("cuda:0")
+    L.load()
+    nz = 10
+    g = syn.domes1000()
+    Ny, Nx = g["thk"].shape
+    t0 = time.time()
+    keys = ("thk", "usurf", "arrhenius", "slidingco", "dX")
+    f32 = {k: torch.as_tensor(g[k], device=dev) for k in keys}
+    f64 = {k: v.double() for k, v in f32.items()}
+    thk = f32["thk"]
+    mask = thk > 0
+    cfg = config.default_cfg()
+    ice = cfg.processes.iceflow
+    ice.method = "emulated"
+    ice.emulator.retrain_freq = 0
+    if args.check_its is not None:
+        ice.emulator.fallback.check_its = args.check_its
+
+    # ------------------------------------------------------------ state --
+    class State:
+        pass
+
+    state = State()
+    for k in keys:
+        setattr(state, k, f32[k])
+    state.it = -1
+    wpath = args.weights
+    if wpath is None:
+        for w in BENCH_WEIGHTS:
+            p = os.path.join(EM.EMULATOR_DIR, w)
+            if os.path.exists(p):
+                wpath = p
+                break
+    ice.emulator.pretrained = True
+    ice.emulator.name = wpath
+    ice.emulator.nbit_init = 0  # no retraining inside the bench
+    iceflow.initialize(cfg, state)
+    dzeta = np.asarray(state.dzeta)
+    log(f"[bench] setup {time.time() - t0:.1f}s; weights {os.path.basename(wpath)}")
+
+    # ------------------------------------------------ float64 minimiser --
+    t = time.time()
+    Ur, Vr, J64, ref_change, op64 = reference_solution(f64, thk, nz, dzeta, args.ref_flat, args.ref_decay)
+    torch.cuda.synchronize()
+    log(f"[bench] float64 reference: {args.ref_flat}+{args.ref_decay} its in {time.time() - t:.1f}s, "
+        f"J={J64:.6f}, change under refinement {ref_change:.2e}")
+
+    # ------------------------------------------------- emulator path --
+    def emulator_step():
+        EM.update_iceflow_emulated(cfg, state)
+        if ice.emulator.fallback.enabled:
+            iceflow.fallback_check(cfg, state)
+
+    for _ in range(args.warmup):
+        emulator_step()
+    torch.cuda.synchronize()
+    state.fallback_count = 0
+    state.fallback_calls = 0
+    with Timer() as tm:
+        for _ in range(args.steps):
+            emulator_step()
+    emu_ms = tm.ms / args.steps
+    emu_fps = 1000.0 / emu_ms
+    emu_err = rel_err(state.U, state.V, Ur, Vr, mask)
+    j_emu = getattr(state, "COST_EMULATED", float("nan"))
+    # inference only (no energy check)
+    with Timer() as tm:
+        for _ in range(args.steps):
+            EM.update_iceflow_emulated(cfg, state)
+    inf_ms = tm.ms / args.steps
+    inf_err = rel_err(state.U, state.V, Ur, Vr, mask)
+    log(f"[bench] emulator step {emu_ms:.3f} ms ({emu_fps:.1f} fields/s), err {emu_err:.4f}; "
+        f"inference only {inf_ms:.3f} ms, err {inf_err:.4f}; fallbacks {state.fallback_count}/{args.steps}")
+
+    # --------------------------------------------------- solver path --
+    op32 = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f32)
+    z = torch.zeros((nz, Ny, Nx), dtype=torch.float32, device=dev)
+    U, V = z, z.clone()
+    opt = LegacyAdamState(lr=1.0)
+    its, err, chunk = 0, 1.0, 250
+    curve = []
+    while its < args.solver_max:
+        r = solve(op32, U, V, chunk, opt, early_stopping=False)
+        U, V, its = r.U, r.V, its + chunk
+        err = rel_err(U, V, Ur, Vr, mask)
+        curve.append((its, round(err, 5)))
+        if err <= 0.01:
+            break
+    n1 = its
+    # bisect the last chunk from its start (deterministic re-run)
+    lo = max(0, n1 - chunk)
+    if err <= 0.01 and lo > 0:
+        opt_b = LegacyAdamState(lr=1.0)
+        rb = solve(op32, z, z.clone(), lo, opt_b, early_stopping=False)
+        Ub, Vb, itb = rb.U, rb.V, lo
+        while itb < n1:
+            rr = solve(op32, Ub, Vb, 25, opt_b, early_stopping=False)
+            Ub, Vb, itb = rr.U, rr.V, itb + 25
+            if rel_err(Ub, Vb, Ur, Vr, mask) <= 0.01:
+                break
+        n1 = itb
+    solver_reached = err <= 0.01
+    # time n1 iterations from a cold start
+    opt_t = LegacyAdamState(lr=1.0)
+    solve(op32, z, z.clone(), 20, opt_t, early_stopping=False)
+    with Timer() as tm:
+        rs = solve(op32, z, z.clone(), n1, LegacyAdamState(lr=1.0), early_stopping=False)
+    solver_ms = tm.ms
+    solver_fps = 1000.0 / solver_ms
+    it_ms = solver_ms / n1
+    log(f"[bench] solver: {n1} its to err <= 1% (reached={solver_reached}) in {solver_ms:.1f} ms "
+        f"({it_ms * 1e3:.1f} us/it) -> {solver_fps:.3f} fields/s; curve {curve[:12]}")
+    # reference early-stopping behaviour (nbitmax large)
+    res_es = solve(op32, z, z.clone(), 20000, LegacyAdamState(lr=1.0), early_stopping=True)
+    es_err = rel_err(res_es.U, res_es.V, Ur, Vr, mask)
+
+    # ------------------------------------------------------- rooflines --
+    cnn = state.iceflow_model
+    fields = EM.planar_fields([state.thk, state.usurf, state.arrhenius, state.slidingco, state.dX])
+    a_in = torch.zeros((1, Ny, Nx, 32), dtype=torch.bfloat16, device=dev)
+    a_in.copy_(torch.randn((1, Ny, Nx, 32), device=dev).to(torch.bfloat16))
+    a_out = torch.empty_like(a_in)
+    nlaunch = 50
+    st = L.stream_ptr()
+    for _ in range(5):
+        L.call("igm_conv3x3_fwd_bf16", L.ptr(a_in), L.ptr(cnn.fwd[0]), L.ptr(cnn.bias(1)), None, 1, Ny, Nx,
+               L.ptr(a_out), st)
+    with Timer() as tm:
+        for _ in range(nlaunch):
+            L.call("igm_conv3x3_fwd_bf16", L.ptr(a_in), L.ptr(cnn.fwd[0]), L.ptr(cnn.bias(1)), None, 1, Ny, Nx,
+                   L.ptr(a_out), st)
+    conv_us = tm.ms * 1e3 / nlaunch
+    conv_flop = 2.0 * 9 * 32 * 32 * Ny * Nx
+    conv_tflops = conv_flop / (conv_us * 1e-6) / 1e12
+    # energy + adjoint (mode 1) and the fused solver iteration (mode 2)
+    gU, gV = torch.empty_like(z), torch.empty_like(z)
+    Uw, Vw = Ur.float().contiguous(), Vr.float().contiguous()
+    for _ in range(5):
+        op32.run_pass(1, Uw, Vw, gU=gU, gV=gV)
+    with Timer() as tm:
+        for _ in range(nlaunch):
+            op32.run_pass(1, Uw, Vw, gU=gU, gV=gV)
+    en_us = tm.ms * 1e3 / nlaunch
+    en_bytes = 176.0 * Ny * Nx
+    en_gbs = en_bytes / (en_us * 1e-6) / 1e9
+    fused_bytes = 496.0 * Ny * Nx
+    fused_gbs = fused_bytes / (it_ms * 1e-3) / 1e9
+    log(f"[bench] conv3x3 {conv_us:.1f} us/launch = {conv_tflops:.1f} TFLOP/s; energy+adjoint {en_us:.1f} us = "
+        f"{en_gbs:.0f} GB/s; solver iteration {it_ms * 1e3:.1f} us = {fused_gbs:.0f} GB/s (496 B/cell)")
+
+    # ----------------------------------------------------- CPU baseline --
+    cpu = None
+    if not args.no_cpu:
+        from oracle import cnn as OC
+        ncore = os.cpu_count() or 1
+        torch.set_num_threads(min(16, ncore))
+        X = torch.stack([torch.as_tensor(g[k]) for k in keys], -1)[None]
+        weights = cnn.get_weights()
+        OC.forward(X[:, :64, :64], weights)
+        tc = time.time()
+        with torch.no_grad():
+            OC.forward(X, weights)
+        cpu_inf_s = time.time() - tc
+        cpu = {"value": 1.0 / cpu_inf_s, "unit": "velocity fields/s", "cores": torch.get_num_threads(),
+               "kind": "port",
+               "sample": "one full 1000x1000 emulator inference (17 conv layers, fp32) of the oracle CNN "
+                         "(torch-CPU restatement of emulate/neural_network.py); TensorFlow is not installed"}
+        log(f"[bench] CPU oracle CNN inference {cpu_inf_s:.2f}s on {torch.get_num_threads()} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(emu_fps, 3),
+        "unit": "velocity fields/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(emu_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": "domes1000: 1000x1000 grid, dx=50 m, Nz=10, 20 seeded domes on a random-field bed; "
+                               "emulated iceflow update (bf16 MFMA CNN 16x3x3x32 + 1x1, fp32 first layer) with "
+                               "float64 energy-gap fallback check vs the fp32 energy solver",
+                   "weights": os.path.basename(wpath),
+                   "fallback_check_its": int(ice.emulator.fallback.check_its)},
+        "roofline": {"bound": "mfma", "achieved": round(conv_tflops, 2), "peak": PEAK_BF16_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(conv_tflops / PEAK_BF16_TFLOPS, 4), "traffic": None,
+                     "kernel": "conv3x3_mfma_kernel<0> (32->32, 1000x1000)", "us_per_launch": round(conv_us, 2)},
+        "cpu_baseline": cpu,
+        "emulator_rel_err_vs_fp64": round(emu_err, 5),
+        "emulator_inference_ms": round(inf_ms, 4),
+        "emulator_inference_rel_err": round(inf_err, 5),
+        "fallbacks": int(state.fallback_count),
+        "solver_fields_per_s": round(solver_fps, 4),
+        "solver_iters_to_1pct": int(n1),
+        "solver_reached_1pct": bool(solver_reached),
+        "solver_us_per_iter": round(it_ms * 1e3, 2),
+        "emulator_vs_solver_speedup": round(emu_fps / solver_fps, 1),
+        "solver_early_stopping": {"iterations": int(res_es.iterations), "rel_err": round(es_err, 5)},
+        "roofline_energy_adjoint": {"bound": "hbm", "achieved": round(en_gbs, 1), "peak": PEAK_HBM_GBS,
+                                    "unit": "GB/s", "frac": round(en_gbs / PEAK_HBM_GBS, 4),
+                                    "bytes_per_cell": 176, "us_per_launch": round(en_us, 2)},
+        "roofline_solver_iteration": {"bound": "hbm", "achieved": round(fused_gbs, 1), "peak": PEAK_HBM_GBS,
+                                      "unit": "GB/s", "frac": round(fused_gbs / PEAK_HBM_GBS, 4),
+                                      "bytes_per_cell": 496},
+        "fp64_reference": {"J": J64, "its": args.ref_flat + args.ref_decay, "change_under_refinement": ref_change},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

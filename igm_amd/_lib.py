@@ -66,6 +66,10 @@ _SIGS = {
                                     VP, VP]),
     "igm_adam_clip_f32": (C.c_int, [VP, VP, VP, VP, I64, VP, I32, VP, F64, VP, VP]),
     "igm_pack_conv_weights": (C.c_int, [VP, VP, I32, VP, I32, VP, VP, VP, VP]),
+    "igm_transport_f32": (C.c_int, [I32, I32, VP, VP, VP, VP, VP, F64, F64, F64, I32, F64, F64, VP, VP, VP,
+                                    VP, VP]),
+    "igm_absmax2_f32": (C.c_int, [VP, VP, I64, VP, VP, VP]),
+    "igm_smb_simple_f32": (C.c_int, [I64, VP, F64, F64, F64, F64, VP, VP]),
 }
 
 _lib = None

@@ -79,7 +79,11 @@ __device__ __forceinline__ float pow_pos(float x, float e) {
 __device__ __forceinline__ double pow_pos(double x, double e) { return pow(x, e); }
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-__device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+// hardware square root (v_sqrt_f32, ~1 ulp) -- not the correctly rounded expansion
+__device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+// clamp of x to [lo, hi] in one v_med3_f32
+__device__ __forceinline__ float clamp_(float x, float lo, float hi) { return __builtin_amdgcn_fmed3f(x, lo, hi); }
+__device__ __forceinline__ double clamp_(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
 
 }  // namespace igm

@@ -44,14 +44,16 @@ constexpr int kDefaultRows = 8;
 // --------------------------------------------------------- coefficients --
 //
 // Plane meaning for the cell with lower-left node (j, i) (S = number of cells):
-//   0  q  = 0.25 * damp / max(H, thr)      damp = 1, or 0.01 where stag4h(C) <= 0
+//   0  q  = dx * damp / max(H, thr)        damp = 1, or 0.01 where stag4h(C) <= 0
 //   1  w  = H * Bs / (p * S)               Bs = stag4h(2 A^(-1/n)), p = 1 + 1/n
 //   2  gx = 0.25 * rho*g*1e-6 * H * sx / S (sx, sy: staggered slope of usurf)
 //   3  gy = 0.25 * rho*g*1e-6 * H * sy / S
 //   4  cs = stag4h(C) / (s * S)            s = 1 + 1/m
 //   5  bx, 6 by: staggered slope of the bed usurf - thk
 // The 0.25 factors absorb the 1/4 of the horizontal node average, which the
-// pass keeps unscaled (sum of the 4 corner nodes).
+// pass keeps unscaled (sum of the 4 corner nodes); dx in q puts the vertical
+// derivative in the pass's units of 1/(4 dx) (dx = dX[b, 0, 0], the scalar
+// spacing of the velocity derivatives, cost_shear.py:113).
 template <typename T>
 __global__ void __launch_bounds__(256)
 energy_coeffs_kernel(const T* __restrict__ thk, const T* __restrict__ usurf,
@@ -103,7 +105,7 @@ energy_coeffs_kernel(const T* __restrict__ thk, const T* __restrict__ usurf,
   const T inv_S = T(1) / (T(ny - 1) * T(nx - 1));
   const T damp = Cs > T(0) ? T(1) : T(0.01);
   const T Hc = H > thr ? H : thr;
-  cb[0 * plane] = T(0.25) * damp / Hc;
+  cb[0 * plane] = dX[base] * damp / Hc;
   cb[1 * plane] = H * Bs * inv_p * inv_S;
   cb[2 * plane] = T(0.25) * rho_g * H * sx * inv_S;
   cb[3 * plane] = T(0.25) * rho_g * H * sy * inv_S;
@@ -113,6 +115,37 @@ energy_coeffs_kernel(const T* __restrict__ thk, const T* __restrict__ usurf,
 }
 
 // ----------------------------------------------------------------- pass --
+//
+// All global accesses of the pass go through buffer resources with 32-bit
+// byte offsets: an offset at or beyond the resource size reads 0 and drops
+// a store, in hardware.  The kernel gives every out-of-grid node (and every
+// lane that does not own its node) the offset kOOB, so the zero boundary of
+// the stencil and the masked stores need no branches.
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr uint32_t kOOB = 0x80000000u;      // > every resource size (host-checked < 2 GiB)
+constexpr int kRsrcWord3 = 0x00020000;      // raw buffer, gfx9 data format
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
+}
+__device__ __forceinline__ float bload(rsrc_t r, uint32_t off, float) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ double bload(rsrc_t r, uint32_t off, double) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  return __hiloint2double((int)v[1], (int)v[0]);
+}
+__device__ __forceinline__ void bstore(float x, rsrc_t r, uint32_t off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, off, 0, 0);
+}
+__device__ __forceinline__ void bstore(double x, rsrc_t r, uint32_t off) {
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  u2 v;
+  v[0] = (unsigned)__double2loint(x);
+  v[1] = (unsigned)__double2hiint(x);
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0);
+}
 
 template <typename T>
 struct PassArgs {
@@ -130,9 +163,10 @@ struct PassArgs {
   const T* hscale;
   double* partials;
   const int32_t* stop;
-  int64_t bs, ks, rs, cs;
-  int64_t plane;
-  int Ny, Nx, rows, nstrips, nrowblk, blocks_per_batch;
+  uint32_t vel_bytes;   // size of every velocity-shaped resource
+  uint32_t coef_bytes;
+  uint32_t bs_bytes;    // batch stride of the velocity layout, bytes
+  int Ny, Nx, plane, rows, nstrips, nrowblk, blocks_per_batch;
   T dz[IGM_MAX_NZ];   // layer thickness dzeta_l
   T idz[IGM_MAX_NZ];  // 1/dzeta_l
   T wz[IGM_MAX_NZ];   // level weight of the gravity term: (dzeta_{k-1} + dzeta_k)/2
@@ -144,7 +178,9 @@ struct PassArgs {
   T alpha, omb1, omb2, eps;  // Adam: alpha, 1-beta1, 1-beta2, epsilon
 };
 
-template <typename T, int NZ, int MODE>
+// INTER = 0: planar (Nz, Ny, Nx) with separate U, V buffers;
+// INTER = 1: node-interleaved Y (Ny, Nx, 2Nz), V = Y + Nz.
+template <typename T, int NZ, int MODE, int INTER>
 __global__ void __launch_bounds__(kPassBlock)
 energy_pass_kernel(const PassArgs<T> a) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -152,6 +188,7 @@ energy_pass_kernel(const PassArgs<T> a) {
   const int b = blockIdx.y;
   double e_sh = 0.0, e_sl = 0.0, e_gr = 0.0;
   const bool stopped = (a.stop != nullptr) && (*a.stop != 0);
+  constexpr uint32_t ES = sizeof(T);
 
   if (!stopped && wid < a.nstrips * a.nrowblk) {
     const int strip = wid % a.nstrips;
@@ -161,26 +198,26 @@ energy_pass_kernel(const PassArgs<T> a) {
     const bool own = (lane >= 1) && (lane <= kStripOut) && col_ok;
     const int j0 = rb * a.rows;
     const int j1 = min(j0 + a.rows, a.Ny);
-    const int64_t boff = (int64_t)b * a.bs + (int64_t)ic * a.cs;
-    const T* __restrict__ Ug = a.U + boff;
-    const T* __restrict__ Vg = a.V + boff;
-    const T* __restrict__ cb = a.coef + (int64_t)b * kPlanes * a.plane + ic;
+    const rsrc_t rU = make_rsrc(a.U, a.vel_bytes), rV = make_rsrc(a.V, a.vel_bytes);
+    const rsrc_t rC = make_rsrc(a.coef, a.coef_bytes);
+    const uint32_t bbase = (uint32_t)b * a.bs_bytes;
+    const uint32_t cbase = (uint32_t)b * (uint32_t)(kPlanes * a.plane) * ES;
+    const uint32_t kstep = INTER ? ES : (uint32_t)a.plane * ES;   // bytes between levels
     const T hh = a.hscale[b];
+    const T hh2 = hh * hh;
 
+    // byte offset of node (row, ic), level 0 -- or kOOB outside the grid
+    auto node_off = [&](int row, bool extra) -> uint32_t {
+      const bool ok = extra && col_ok && row >= 0 && row < a.Ny;
+      const uint32_t e = (uint32_t)(row * a.Nx + ic) * (INTER ? 2u * NZ : 1u);
+      return ok ? bbase + e * ES : kOOB;
+    };
     auto load_row = [&](T (&u)[NZ], T (&v)[NZ], int row) {
-      if (col_ok && row >= 0 && row < a.Ny) {
-        const int64_t o = (int64_t)row * a.rs;
+      const uint32_t o = node_off(row, true);
 #pragma unroll
-        for (int k = 0; k < NZ; ++k) {
-          u[k] = Ug[o + k * a.ks];
-          v[k] = Vg[o + k * a.ks];
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < NZ; ++k) {
-          u[k] = T(0);
-          v[k] = T(0);
-        }
+      for (int k = 0; k < NZ; ++k) {
+        u[k] = bload(rU, o + k * kstep, T());
+        v[k] = bload(rV, o + k * kstep, T());
       }
     };
 
@@ -197,82 +234,59 @@ energy_pass_kernel(const PassArgs<T> a) {
       T un[NZ], vn[NZ];
       load_row(un, vn, jc + 2);  // prefetch for the next step
 
-      T q = 0, w = 0, gx = 0, gy = 0, csl = 0, bx = 0, by = 0;
-      if (col_ok && jc >= 0) {
-        const T* c = cb + (int64_t)jc * a.Nx;
-        q = c[0];
-        w = c[a.plane];
-        gx = c[2 * a.plane];
-        gy = c[3 * a.plane];
-        csl = c[4 * a.plane];
-        bx = c[5 * a.plane];
-        by = c[6 * a.plane];
-      }
+      const uint32_t co = (col_ok && jc >= 0) ? cbase + (uint32_t)(jc * a.Nx + ic) * ES : kOOB;
+      const uint32_t ps = (uint32_t)a.plane * ES;
+      const T q = bload(rC, co, T()), w = bload(rC, co + ps, T());
+      const T gx = bload(rC, co + 2 * ps, T()), gy = bload(rC, co + 3 * ps, T());
+      const T csl = bload(rC, co + 4 * ps, T());
+      const T bx = bload(rC, co + 5 * ps, T()), by = bload(rC, co + 6 * ps, T());
       const bool emit_row = own && (jc >= j0);
-      const int64_t onode = (int64_t)jc * a.rs;  // node (jc, ic), level 0
-      T mu[NZ], vu_[NZ], mv[NZ], vv_[NZ];
-      if constexpr (MODE == 2) {
-        if (emit_row) {
-          T* mUg = a.mU + (int64_t)b * a.bs + (int64_t)ic * a.cs + onode;
-          T* vUg = a.vU + (int64_t)b * a.bs + (int64_t)ic * a.cs + onode;
-          T* mVg = a.mV + (int64_t)b * a.bs + (int64_t)ic * a.cs + onode;
-          T* vVg = a.vV + (int64_t)b * a.bs + (int64_t)ic * a.cs + onode;
-#pragma unroll
-          for (int k = 0; k < NZ; ++k) {
-            mu[k] = mUg[k * a.ks];
-            vu_[k] = vUg[k * a.ks];
-            mv[k] = mVg[k * a.ks];
-            vv_[k] = vVg[k * a.ks];
-          }
-        }
-      }
+      const uint32_t on = node_off(jc, emit_row);  // output node (jc, ic), masked
 
-      // Scatter of the adjoint of level k to the 4 corner nodes, then output
-      // of node (jc, ic), level k.
+      // scatter of the adjoint of level k to the 4 corner nodes, then the
+      // output of node (jc, ic), level k (masked stores)
       auto emit = [&](int k, T gux, T guy, T gsu, T gvx, T gvy, T gsv) {
+        const uint32_t ok = on + k * kstep;
         {
-          const T qx = gux * hh, qy = guy * hh;
-          const T A = gsu - qx, B = gsu + qx;
-          const T c00 = A - qy, c10 = A + qy, c01 = B - qy, c11 = B + qy;
+          const T A = gsu - gux, B = gsu + gux;
+          const T c00 = A - guy, c10 = A + guy, c01 = B - guy, c11 = B + guy;
           const T g = cu[k] + c00 + lane_from_left(c01);
           cu[k] = c10 + lane_from_left(c11);
-          if (emit_row) {
-            if constexpr (MODE == 1) {
-              a.gU[(int64_t)b * a.bs + (int64_t)ic * a.cs + onode + k * a.ks] = g;
-            } else if constexpr (MODE == 2) {
-              T m = mu[k], v = vu_[k];
-              m += (g - m) * a.omb1;
-              v += (g * g - v) * a.omb2;
-              const int64_t o = (int64_t)b * a.bs + (int64_t)ic * a.cs + onode + k * a.ks;
-              a.Uo[o] = up[k] - (m * a.alpha) / (sqrt_(v) + a.eps);
-              a.mU[o] = m;
-              a.vU[o] = v;
-            }
+          if constexpr (MODE == 1) {
+            bstore(g, make_rsrc(a.gU, a.vel_bytes), ok);
+          } else if constexpr (MODE == 2) {
+            const rsrc_t rm = make_rsrc(a.mU, a.vel_bytes), rv = make_rsrc(a.vU, a.vel_bytes);
+            T m = bload(rm, ok, T()), v = bload(rv, ok, T());
+            m += (g - m) * a.omb1;
+            v += (g * g - v) * a.omb2;
+            bstore(up[k] - (m * a.alpha) * rcp(sqrt_(v) + a.eps), make_rsrc(a.Uo, a.vel_bytes), ok);
+            bstore(m, rm, ok);
+            bstore(v, rv, ok);
           }
         }
         {
-          const T qx = gvx * hh, qy = gvy * hh;
-          const T A = gsv - qx, B = gsv + qx;
-          const T c00 = A - qy, c10 = A + qy, c01 = B - qy, c11 = B + qy;
+          const T A = gsv - gvx, B = gsv + gvx;
+          const T c00 = A - gvy, c10 = A + gvy, c01 = B - gvy, c11 = B + gvy;
           const T g = cv[k] + c00 + lane_from_left(c01);
           cv[k] = c10 + lane_from_left(c11);
-          if (emit_row) {
-            if constexpr (MODE == 1) {
-              a.gV[(int64_t)b * a.bs + (int64_t)ic * a.cs + onode + k * a.ks] = g;
-            } else if constexpr (MODE == 2) {
-              T m = mv[k], v = vv_[k];
-              m += (g - m) * a.omb1;
-              v += (g * g - v) * a.omb2;
-              const int64_t o = (int64_t)b * a.bs + (int64_t)ic * a.cs + onode + k * a.ks;
-              a.Vo[o] = vp[k] - (m * a.alpha) / (sqrt_(v) + a.eps);
-              a.mV[o] = m;
-              a.vV[o] = v;
-            }
+          if constexpr (MODE == 1) {
+            bstore(g, make_rsrc(a.gV, a.vel_bytes), ok);
+          } else if constexpr (MODE == 2) {
+            const rsrc_t rm = make_rsrc(a.mV, a.vel_bytes), rv = make_rsrc(a.vV, a.vel_bytes);
+            T m = bload(rm, ok, T()), v = bload(rv, ok, T());
+            m += (g - m) * a.omb1;
+            v += (g * g - v) * a.omb2;
+            bstore(vp[k] - (m * a.alpha) * rcp(sqrt_(v) + a.eps), make_rsrc(a.Vo, a.vel_bytes), ok);
+            bstore(m, rm, ok);
+            bstore(v, rv, ok);
           }
         }
       };
 
-      // level sweep; layer l = k-1 is handled once level k is known
+      // Level sweep; layer l = k-1 is handled once level k is known.  The
+      // horizontal derivatives are kept in units of 1/(4 dx) (ux = 4 dx *
+      // dU/dx at the level), the vertical ones likewise (q holds damp*dx/Hc);
+      // hh2 = 1/(4 dx)^2 restores the strain rate where it is needed.
       T p_ux = 0, p_uy = 0, p_vx = 0, p_vy = 0, p_su = 0, p_sv = 0;  // level k-1
       T g_ux = 0, g_uy = 0, g_vx = 0, g_vy = 0, g_su = 0, g_sv = 0;  // its adjoint
       T c_sh = 0, c_gr = 0, c_sl = 0;
@@ -282,11 +296,11 @@ energy_pass_kernel(const PassArgs<T> a) {
         const T sv = vp[k] + vc[k], dv = vc[k] - vp[k];
         const T su_r = lane_from_right(su), du_r = lane_from_right(du);
         const T sv_r = lane_from_right(sv), dv_r = lane_from_right(dv);
-        const T ux = (su_r - su) * hh, uy = (du + du_r) * hh;  // half of the level derivative
-        const T vx = (sv_r - sv) * hh, vy = (dv + dv_r) * hh;
-        const T sU = su + su_r, sV = sv + sv_r;                 // 4 x node average
+        const T ux = su_r - su, uy = du + du_r;
+        const T vx = sv_r - sv, vy = dv + dv_r;
+        const T sU = su + su_r, sV = sv + sv_r;  // 4 x node average
         T n_ux = 0, n_uy = 0, n_vx = 0, n_vy = 0;
-        T n_su = a.wz[k] * gx, n_sv = a.wz[k] * gy;             // gravity (linear)
+        T n_su = a.wz[k] * gx, n_sv = a.wz[k] * gy;  // gravity (linear)
         c_gr += a.wz[k] * (gx * sU + gy * sV);
         if (k == 0) {  // Weertman sliding on the basal level
           const T ub = T(0.25) * sU, vb = T(0.25) * sV;
@@ -308,16 +322,17 @@ energy_pass_kernel(const PassArgs<T> a) {
           const T Dvx = dVdx - dVdz * bx, Dvy = dVdy - dVdz * by;
           const T Exy = T(0.5) * (Dvx + Duy);
           const T Ezz = -(Dux + Dvy);
-          const T sr2 = T(0.5) * (Dux * Dux + T(2) * (Exy * Exy) + Dvy * Dvy + Ezz * Ezz) +
-                        T(0.25) * (dUdz * dUdz + dVdz * dVdz);
-          const bool in_rng = (sr2 >= a.min_sr2) && (sr2 <= a.max_sr2);
-          const T sc = (sr2 < a.min_sr2 ? a.min_sr2 : (sr2 > a.max_sr2 ? a.max_sr2 : sr2)) + a.regu2;
+          const T sr2 = hh2 * (T(0.5) * (Dux * Dux + T(2) * (Exy * Exy) + Dvy * Dvy + Ezz * Ezz) +
+                               T(0.25) * (dUdz * dUdz + dVdz * dVdz));
+          const T sclip = clamp_(sr2, a.min_sr2, a.max_sr2);
+          const bool in_rng = (sclip == sr2);  // the clip passes its gradient inside the range
+          const T sc = sclip + a.regu2;
           const T pw = pow_pos(sc, a.e_glen);
           const T wl = a.dz[l] * pw;
           c_sh += wl * sr2;
-          // adjoint of the layer
+          // adjoint of the layer (in the scaled variables)
           const T ratio = in_rng ? a.e_glen * sr2 * rcp(sc) : T(0);
-          const T G = w * wl * (T(1) + ratio);
+          const T G = w * wl * (T(1) + ratio) * hh2;
           const T gDux = G * (Dux - Ezz), gDvy = G * (Dvy - Ezz), gDxy = G * Exy;
           const T gdUdz = T(0.5) * G * dUdz - gDux * bx - gDxy * by;
           const T gdVdz = T(0.5) * G * dVdz - gDxy * bx - gDvy * by;
@@ -380,12 +395,21 @@ energy_pass_kernel(const PassArgs<T> a) {
   }
 }
 
+int rows_per_wave() {
+  static int rows = [] {
+    const char* e = getenv("IGM_ENERGY_ROWS");
+    const int r = e ? atoi(e) : kDefaultRows;
+    return (r >= 1 && r <= 1024) ? r : kDefaultRows;
+  }();
+  return rows;
+}
+
 struct PassGeom {
   int rows, nstrips, nrowblk, blocks;
 };
 inline PassGeom pass_geom(int Ny, int Nx) {
   PassGeom g;
-  g.rows = kDefaultRows;
+  g.rows = rows_per_wave();
   g.nstrips = (Nx + kStripOut - 1) / kStripOut;
   g.nrowblk = (Ny + g.rows - 1) / g.rows;
   const int waves = g.nstrips * g.nrowblk;
@@ -482,12 +506,12 @@ int coeffs_impl(const T* thk, const T* usurf, const T* arrh, const T* slid, cons
   return 0;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int INTER>
 void launch_pass_nz(int nz, dim3 grid, hipStream_t st, const PassArgs<T>& a, bool* ok) {
   *ok = true;
   switch (nz) {
 #define IGM_NZ_CASE(N) \
-  case N: hipLaunchKernelGGL((energy_pass_kernel<T, N, MODE>), grid, dim3(kPassBlock), 0, st, a); break;
+  case N: hipLaunchKernelGGL((energy_pass_kernel<T, N, MODE, INTER>), grid, dim3(kPassBlock), 0, st, a); break;
     IGM_NZ_CASE(2)
     IGM_NZ_CASE(3)
     IGM_NZ_CASE(4)
@@ -519,12 +543,27 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
     IGM_CHECK_ARG(Uo && Vo && mU && vU && mV && vV && adam, "mode 2 needs Uout, Vout, m, v and adam");
     IGM_CHECK_ARG(Uo != U && Vo != V, "Uout/Vout must not alias U/V");
   }
+  // supported layouts: planar (Nz, Ny, Nx) or node-interleaved (Ny, Nx, 2 Nz)
+  const int64_t plane = (int64_t)Ny * Nx;
+  int inter = -1;
+  if (cs == 1 && rs == Nx && ks == plane) inter = 0;
+  if (ks == 1 && cs == 2 * nz && rs == (int64_t)Nx * 2 * nz && V == U + nz) inter = 1;
+  IGM_CHECK_ARG(inter >= 0, "unsupported velocity layout (need planar (Nz,Ny,Nx) or interleaved (Ny,Nx,2Nz))");
+  const int64_t per = inter ? plane * 2 * nz : plane * nz;  // elements of one batch entry
+  if (nbatch > 1) IGM_CHECK_ARG(bs >= per, "batch stride smaller than one entry");
+  const int64_t span = (int64_t)(nbatch - 1) * (nbatch > 1 ? bs : 0) + per;
+  const int64_t vel_bytes = span * (int64_t)sizeof(T);
+  const int64_t coef_bytes = (int64_t)nbatch * kPlanes * plane * (int64_t)sizeof(T);
+  IGM_CHECK_ARG(vel_bytes < (int64_t)kOOB && coef_bytes < (int64_t)kOOB,
+                "problem too large for 32-bit buffer offsets (2 GiB per buffer)");
   PassArgs<T> a{};
   a.U = U; a.V = V; a.gU = gU; a.gV = gV; a.Uo = Uo; a.Vo = Vo;
   a.mU = mU; a.vU = vU; a.mV = mV; a.vV = vV;
   a.coef = coef; a.hscale = hscale; a.partials = partials; a.stop = stop;
-  a.bs = bs; a.ks = ks; a.rs = rs; a.cs = cs;
-  a.plane = (int64_t)Ny * Nx;
+  a.vel_bytes = (uint32_t)vel_bytes;
+  a.coef_bytes = (uint32_t)coef_bytes;
+  a.bs_bytes = (uint32_t)((nbatch > 1 ? bs : 0) * (int64_t)sizeof(T));
+  a.plane = (int)plane;
   a.Ny = Ny; a.Nx = Nx;
   const PassGeom g = pass_geom(Ny, Nx);
   a.rows = g.rows; a.nstrips = g.nstrips; a.nrowblk = g.nrowblk; a.blocks_per_batch = g.blocks;
@@ -559,9 +598,11 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
   dim3 grid(g.blocks, nbatch);
   bool ok = false;
   hipStream_t st = (hipStream_t)stream;
-  if (mode == 0) launch_pass_nz<T, 0>(nz, grid, st, a, &ok);
-  else if (mode == 1) launch_pass_nz<T, 1>(nz, grid, st, a, &ok);
-  else launch_pass_nz<T, 2>(nz, grid, st, a, &ok);
+#define IGM_MODE_LAYOUT(M, I) \
+  if (mode == M && inter == I) launch_pass_nz<T, M, I>(nz, grid, st, a, &ok);
+  IGM_MODE_LAYOUT(0, 0) IGM_MODE_LAYOUT(1, 0) IGM_MODE_LAYOUT(2, 0)
+  IGM_MODE_LAYOUT(0, 1) IGM_MODE_LAYOUT(1, 1) IGM_MODE_LAYOUT(2, 1)
+#undef IGM_MODE_LAYOUT
   IGM_CHECK_ARG(ok, "nz not compiled (supported: 2,3,4,5,6,8,10,12,16,20)");
   IGM_CHECK_LAUNCH();
   return 0;

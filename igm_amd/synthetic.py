@@ -91,25 +91,25 @@ def dome100():
     return _pack(bed, H, DEFAULT_A, DEFAULT_C, dx)
 
 
-def domes400():
+def domes400(seed=1):
     shape, dx = (400, 400), 50.0
-    bed = _bed(shape, dx, seed=1)
-    H = random_domes(shape, dx, 4, seed=1)
-    A = np.random.default_rng(101).uniform(50.0, 100.0, shape)
+    bed = _bed(shape, dx, seed=seed)
+    H = random_domes(shape, dx, 4, seed=seed)
+    A = np.random.default_rng(100 + seed).uniform(50.0, 100.0, shape)
     return _pack(bed, H, A, DEFAULT_C, dx)
 
 
-def domes1000():
+def domes1000(seed=2):
     shape, dx = (1000, 1000), 50.0
-    bed = _bed(shape, dx, seed=2)
-    H = random_domes(shape, dx, 20, seed=2)
+    bed = _bed(shape, dx, seed=seed)
+    H = random_domes(shape, dx, 20, seed=seed)
     return _pack(bed, H, DEFAULT_A, DEFAULT_C, dx)
 
 
-def domes1500():
+def domes1500(seed=4):
     shape, dx = (1500, 1500), 50.0
-    bed = _bed(shape, dx, seed=4)
-    H = random_domes(shape, dx, 40, seed=4)
+    bed = _bed(shape, dx, seed=seed)
+    H = random_domes(shape, dx, 40, seed=seed)
     return _pack(bed, H, DEFAULT_A, DEFAULT_C, dx)
 
 

@@ -219,6 +219,29 @@ int igm_pack_conv_weights(const float* w, const int64_t* offs3x3, int32_t nlayer
                           const float* w_head, int32_t cout, void* fwd, void* dgrad, void* head,
                           void* stream);
 
+/* ----------------------------------------------------------- transport -- */
+
+/* Fused thickness update of the transport step (thk.py:27-54 with
+ * compute_divflux_slope_limiter.py:9-65): slope-limited upwind flux
+ * divergence of (ubar, vbar) * thk, thk_out = max(thk + dt (smb - div), 0),
+ * lsurf = max(topg, -ratio_density thk_out + sealevel), usurf = lsurf + thk_out.
+ * All fields (ny, nx) fp32; smb, lsurf, divflux nullable; thk_out must not
+ * alias thk.  slope_type: 0 godunov, 1 minmod, 2 superbee. */
+int igm_transport_f32(int32_t ny, int32_t nx, const float* ubar, const float* vbar,
+                      const float* thk, const float* smb, const float* topg, double dx, double dy,
+                      double dt, int32_t slope_type, double ratio_density, double sealevel,
+                      float* thk_out, float* lsurf, float* usurf, float* divflux, void* stream);
+
+/* out[0] = max(max|a|, max|b|) over n values (time.py:34-49, the CFL speed).
+ * workspace: 1024 floats. */
+int igm_absmax2_f32(const float* a, const float* b, int64_t n, float* workspace, float* out,
+                    void* stream);
+
+/* smb = clip((usurf - ela) * (usurf < ela ? gradabl : gradacc), -100, maxacc)
+ * (smb_simple.py:40-42). */
+int igm_smb_simple_f32(int64_t n, const float* usurf, double gradabl, double gradacc, double ela,
+                       double maxacc, float* smb, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
