@@ -62,21 +62,29 @@ class Timer:
         self.ms = self.a.elapsed_time(self.b)
 
 
-def reference_solution(f64, thk, nz, dzeta, its_flat, its_decay):
-    """float64 minimiser: legacy Adam lr = 1, then a geometric decay of the step."""
+def adam_schedule(lr0, n_flat, n_decay, lr_end=1e-3):
+    """Step schedule of the tuned solver: lr0 for n_flat iterations, then a
+    geometric decay to lr_end over n_decay iterations."""
+    d = (lr_end / lr0) ** (1.0 / max(1, n_decay))
+    return lambda i: lr0 if i < n_flat else max(lr_end, lr0 * d ** (i - n_flat))
+
+
+def reference_solution(f32, f64, thk, nz, dzeta, n_flat=8000, n_decay=20000, n_refine=2000):
+    """float64 minimiser: float32 Adam, lr 10 then a geometric decay to 1e-3
+    (converges on this workload: runs of 14k and 28k iterations agree to 3e-3),
+    then float64 Adam from that point with the step decaying 1e-3 -> 1e-5."""
     from igm_amd.energy import EnergyOperator
     from igm_amd.solver import LegacyAdamState, solve
     from oracle.energy import DEFAULT_PHYSICS
-    op = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f64, dtype=torch.float64)
-    z = torch.zeros((nz,) + tuple(thk.shape), dtype=torch.float64, device=thk.device)
-    opt = LegacyAdamState(lr=1.0)
-    decay = (1e-3) ** (1.0 / max(1, its_decay))
-    sched = lambda i: 1.0 if i < its_flat else max(1e-3, decay ** (i - its_flat))
-    r = solve(op, z, z.clone(), its_flat + its_decay, opt, early_stopping=False, lr_schedule=sched)
-    # convergence indicator: change over a further float64 refinement
-    r2 = solve(op, r.U, r.V, max(100, its_decay // 10), opt, early_stopping=False, lr_schedule=lambda i: 1e-3)
+    op32 = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f32)
+    z = torch.zeros((nz,) + tuple(thk.shape), dtype=torch.float32, device=thk.device)
+    r = solve(op32, z, z.clone(), n_flat + n_decay, LegacyAdamState(lr=1.0), early_stopping=False,
+              lr_schedule=adam_schedule(10.0, n_flat, n_decay))
+    op64 = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f64, dtype=torch.float64)
+    r2 = solve(op64, r.U.double(), r.V.double(), n_refine, LegacyAdamState(lr=1.0), early_stopping=False,
+               lr_schedule=adam_schedule(1e-3, 0, n_refine, lr_end=1e-5))
     change = rel_err(r.U, r.V, r2.U, r2.V, thk > 0)
-    return r2.U, r2.V, float(r2.costs[-1]), change, op
+    return r2.U, r2.V, float(r2.costs[-1]), change, op64
 
 
 def main():
@@ -84,8 +92,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--ref-flat", type=int, default=3000)
-    ap.add_argument("--ref-decay", type=int, default=6000)
+    ap.add_argument("--ref-flat", type=int, default=8000)
+    ap.add_argument("--ref-decay", type=int, default=20000)
     ap.add_argument("--solver-max", type=int, default=20000)
     ap.add_argument("--check-its", type=int, default=None, help="fallback check iterations (cfg default if unset)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -145,10 +153,10 @@ def main():
 
     # ------------------------------------------------ float64 minimiser --
     t = time.time()
-    Ur, Vr, J64, ref_change, op64 = reference_solution(f64, thk, nz, dzeta, args.ref_flat, args.ref_decay)
+    Ur, Vr, J64, ref_change, op64 = reference_solution(f32, f64, thk, nz, dzeta, args.ref_flat, args.ref_decay)
     torch.cuda.synchronize()
     log(f"[bench] float64 reference: {args.ref_flat}+{args.ref_decay} its in {time.time() - t:.1f}s, "
-        f"J={J64:.6f}, change under refinement {ref_change:.2e}")
+        f"J={J64:.6f}, change under float64 refinement {ref_change:.2e}")
 
     # ------------------------------------------------- emulator path --
     def emulator_step():
@@ -180,41 +188,40 @@ def main():
     # --------------------------------------------------- solver path --
     op32 = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f32)
     z = torch.zeros((nz, Ny, Nx), dtype=torch.float32, device=dev)
+    # (a) the reference's own settings: Adam lr = step_size = 1, no decay
     U, V = z, z.clone()
     opt = LegacyAdamState(lr=1.0)
-    its, err, chunk = 0, 1.0, 250
-    curve = []
-    while its < args.solver_max:
+    its, curve = 0, []
+    for chunk in [1000] * 4 + [2000] * 3 + [4000] * 2:
         r = solve(op32, U, V, chunk, opt, early_stopping=False)
         U, V, its = r.U, r.V, its + chunk
-        err = rel_err(U, V, Ur, Vr, mask)
-        curve.append((its, round(err, 5)))
-        if err <= 0.01:
+        curve.append((its, round(rel_err(U, V, Ur, Vr, mask), 4)))
+        if its >= args.solver_max:
             break
-    n1 = its
-    # bisect the last chunk from its start (deterministic re-run)
-    lo = max(0, n1 - chunk)
-    if err <= 0.01 and lo > 0:
-        opt_b = LegacyAdamState(lr=1.0)
-        rb = solve(op32, z, z.clone(), lo, opt_b, early_stopping=False)
-        Ub, Vb, itb = rb.U, rb.V, lo
-        while itb < n1:
-            rr = solve(op32, Ub, Vb, 25, opt_b, early_stopping=False)
-            Ub, Vb, itb = rr.U, rr.V, itb + 25
-            if rel_err(Ub, Vb, Ur, Vr, mask) <= 0.01:
-                break
-        n1 = itb
-    solver_reached = err <= 0.01
-    # time n1 iterations from a cold start
-    opt_t = LegacyAdamState(lr=1.0)
-    solve(op32, z, z.clone(), 20, opt_t, early_stopping=False)
+    ref_settings = {"lr": 1.0, "error_curve": curve}
+    # (b) the same method with a decaying step: the shortest schedule of the
+    # family lr0 = 10, 30 % flat + 70 % geometric decay to 1e-3 that ends <= 1 %
+    best = None
+    for total in (1500, 2500, 4000, 6000, 9000, 14000, 20000):
+        n_flat = int(0.3 * total)
+        sched = adam_schedule(10.0, n_flat, total - n_flat)
+        r = solve(op32, z, z.clone(), total, LegacyAdamState(lr=1.0), early_stopping=False, lr_schedule=sched)
+        e = rel_err(r.U, r.V, Ur, Vr, mask)
+        log(f"[bench] tuned solver schedule {total} its: rel err {e:.4f}")
+        if e <= 0.01:
+            best = (total, n_flat, e)
+            break
+    solver_reached = best is not None
+    n1, n_flat1, solver_err = best if best else (total, n_flat, e)
+    sched = adam_schedule(10.0, n_flat1, n1 - n_flat1)
+    solve(op32, z, z.clone(), 20, LegacyAdamState(lr=1.0), early_stopping=False)
     with Timer() as tm:
-        rs = solve(op32, z, z.clone(), n1, LegacyAdamState(lr=1.0), early_stopping=False)
+        rs = solve(op32, z, z.clone(), n1, LegacyAdamState(lr=1.0), early_stopping=False, lr_schedule=sched)
     solver_ms = tm.ms
     solver_fps = 1000.0 / solver_ms
     it_ms = solver_ms / n1
-    log(f"[bench] solver: {n1} its to err <= 1% (reached={solver_reached}) in {solver_ms:.1f} ms "
-        f"({it_ms * 1e3:.1f} us/it) -> {solver_fps:.3f} fields/s; curve {curve[:12]}")
+    log(f"[bench] solver (tuned schedule): {n1} its to err {solver_err:.4f} (<= 1%: {solver_reached}) in "
+        f"{solver_ms:.1f} ms ({it_ms * 1e3:.1f} us/it) -> {solver_fps:.3f} fields/s; reference settings {curve}")
     # reference early-stopping behaviour (nbitmax large)
     res_es = solve(op32, z, z.clone(), 20000, LegacyAdamState(lr=1.0), early_stopping=True)
     es_err = rel_err(res_es.U, res_es.V, Ur, Vr, mask)
@@ -299,8 +306,11 @@ def main():
         "emulator_inference_rel_err": round(inf_err, 5),
         "fallbacks": int(state.fallback_count),
         "solver_fields_per_s": round(solver_fps, 4),
+        "solver_schedule": {"lr0": 10.0, "flat": int(n_flat1), "decay_to_1e-3": int(n1 - n_flat1),
+                            "rel_err": round(solver_err, 5)},
         "solver_iters_to_1pct": int(n1),
         "solver_reached_1pct": bool(solver_reached),
+        "solver_reference_settings": ref_settings,
         "solver_us_per_iter": round(it_ms * 1e3, 2),
         "emulator_vs_solver_speedup": round(emu_fps / solver_fps, 1),
         "solver_early_stopping": {"iterations": int(res_es.iterations), "rel_err": round(es_err, 5)},

@@ -114,31 +114,24 @@ conv_in_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
 // MODE 1 (dgrad): out = conv * (aux > 0 ? 1 : 0.01), aux = activation of the
 // layer below (its sign is the sign of its pre-activation).
 // Block = 4 waves; tile = 4 output rows x 64 pixels; wave w owns row y0 + w.
+// Persistent: block k walks tiles [k*per, (k+1)*per) of the tile list ordered
+// with the row-block index fastest, so a block's tiles are vertically
+// adjacent (their halo rows meet in the same L2) and the 18 weight
+// fragments are loaded once per block.
 template <int MODE>
 __global__ void __launch_bounds__(256)
 conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
                     const float* __restrict__ bias, const bf16* __restrict__ aux,
-                    const int32_t* __restrict__ ext, int H, int W, bf16* __restrict__ out) {
+                    const int32_t* __restrict__ ext, int B, int H, int W, int per,
+                    bf16* __restrict__ out) {
   __shared__ __attribute__((aligned(16))) bf16 tile[kTileH + 2][kTileW + 2][kPxStride];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int x0 = blockIdx.x * kTileW, y0 = blockIdx.y * kTileH, b = blockIdx.z;
-  int ny, nx;
-  extent_of(ext, b, H, W, ny, nx);
-  const int64_t img = (int64_t)b * H * W;
-  const int y = y0 + wave;
-
-  if (y0 >= ny || x0 >= nx) {  // tile outside the valid region: zeros
-    if (y < H) {
-      const int x = x0 + lane;
-      if (x < W) {
-        uint4 z = {0, 0, 0, 0};
-        uint4* o = reinterpret_cast<uint4*>(out + ((img + (int64_t)y * W + x) * kC));
-#pragma unroll
-        for (int q = 0; q < 4; ++q) o[q] = z;
-      }
-    }
-    return;
-  }
+  const int nty = (H + kTileH - 1) / kTileH;
+  const int ntx = (W + kTileW - 1) / kTileW;
+  const int64_t ntiles = (int64_t)B * ntx * nty;
+  const int64_t t0 = (int64_t)blockIdx.x * per;
+  if (t0 >= ntiles) return;
+  const int64_t t1 = t0 + per < ntiles ? t0 + per : ntiles;
 
   // weight fragments: A operand (16 out-channels x 32 in-channels per tap)
   bf16x8 wf[9][2];
@@ -147,65 +140,109 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb)
       wf[t][cb] = *reinterpret_cast<const bf16x8*>(wt + ((t * kC + cb * 16 + (lane & 15)) * kC + 8 * (lane >> 4)));
-
-  // stage input rows y0-1 .. y0+4, pixels x0-1 .. x0+64 (zero outside)
-  constexpr int kChunks = (kTileH + 2) * (kTileW + 2) * 4;
-  for (int c = tid; c < kChunks; c += 256) {
-    const int chunk = c & 3;
-    const int px = (c >> 2) % (kTileW + 2);
-    const int row = (c >> 2) / (kTileW + 2);
-    const int yy = y0 - 1 + row, xx = x0 - 1 + px;
-    uint4 v = {0, 0, 0, 0};
-    if (yy >= 0 && yy < ny && xx >= 0 && xx < nx)
-      v = *reinterpret_cast<const uint4*>(in + (img + (int64_t)yy * W + xx) * kC + chunk * 8);
-    *reinterpret_cast<uint4*>(&tile[row][px][chunk * 8]) = v;
-  }
-  __syncthreads();
-
-  f32x4 acc[4][2];
+  float bias_r[2][4];
+  if constexpr (MODE == 0) {
 #pragma unroll
-  for (int pb = 0; pb < 4; ++pb)
+    for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) acc[pb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-#pragma unroll
-  for (int pb = 0; pb < 4; ++pb) {
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-      const int r = t / 3, s = t % 3;
-      const bf16x8 bfr =
-          *reinterpret_cast<const bf16x8*>(&tile[wave + r][pb * 16 + (lane & 15) + s][8 * (lane >> 4)]);
-      acc[pb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[t][0], bfr, acc[pb][0], 0, 0, 0);
-      acc[pb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[t][1], bfr, acc[pb][1], 0, 0, 0);
-    }
+      for (int e = 0; e < 4; ++e) bias_r[cb][e] = bias[cb * 16 + 4 * (lane >> 4) + e];
   }
 
-  if (y >= H) return;
+  for (int64_t ti = t0; ti < t1; ++ti) {
+    const int ty = (int)(ti % nty);
+    const int tx = (int)((ti / nty) % ntx);
+    const int b = (int)(ti / ((int64_t)nty * ntx));
+    const int x0 = tx * kTileW, y0 = ty * kTileH;
+    int ny, nx;
+    extent_of(ext, b, H, W, ny, nx);
+    const int64_t img = (int64_t)b * H * W;
+    const int y = y0 + wave;
+
+    if (y0 >= ny || x0 >= nx) {  // tile outside the valid region: zeros
+      if (y < H) {
+        const int x = x0 + lane;
+        if (x < W) {
+          uint4 z = {0, 0, 0, 0};
+          uint4* o = reinterpret_cast<uint4*>(out + ((img + (int64_t)y * W + x) * kC));
 #pragma unroll
-  for (int pb = 0; pb < 4; ++pb) {
-    const int x = x0 + pb * 16 + (lane & 15);
-    if (x >= W) continue;
-    const bool valid = (y < ny) && (x < nx);
-    const int64_t po = (img + (int64_t)y * W + x) * kC;
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-      const int co = cb * 16 + 4 * (lane >> 4);
-      float v[4];
-      if constexpr (MODE == 0) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = leaky(acc[pb][cb][e] + bias[co + e]);
-      } else {
-        const uint2 g2 = *reinterpret_cast<const uint2*>(aux + po + co);
-        const bf16* g = reinterpret_cast<const bf16*>(&g2);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = acc[pb][cb][e] * (bf2f(g[e]) > 0.f ? 1.f : 0.01f);
+          for (int q = 0; q < 4; ++q) o[q] = z;
+        }
       }
-      bf16 h[4];
+      continue;  // block-uniform
+    }
+
+    __syncthreads();  // the previous tile's fragment reads are done
+    // stage input rows y0-1 .. y0+4, pixels x0-1 .. x0+64 (zero outside)
+    constexpr int kChunks = (kTileH + 2) * (kTileW + 2) * 4;
+    for (int c = tid; c < kChunks; c += 256) {
+      const int chunk = c & 3;
+      const int px = (c >> 2) % (kTileW + 2);
+      const int row = (c >> 2) / (kTileW + 2);
+      const int yy = y0 - 1 + row, xx = x0 - 1 + px;
+      uint4 v = {0, 0, 0, 0};
+      if (yy >= 0 && yy < ny && xx >= 0 && xx < nx)
+        v = *reinterpret_cast<const uint4*>(in + (img + (int64_t)yy * W + xx) * kC + chunk * 8);
+      *reinterpret_cast<uint4*>(&tile[row][px][chunk * 8]) = v;
+    }
+    __syncthreads();
+
+    f32x4 acc[4][2];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) h[e] = valid ? f2bf(v[e]) : f2bf(0.f);
-      *reinterpret_cast<uint2*>(out + po + co) = *reinterpret_cast<const uint2*>(h);
+    for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) acc[pb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+    for (int pb = 0; pb < 4; ++pb) {
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        const int r = t / 3, s = t % 3;
+        const bf16x8 bfr =
+            *reinterpret_cast<const bf16x8*>(&tile[wave + r][pb * 16 + (lane & 15) + s][8 * (lane >> 4)]);
+        acc[pb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[t][0], bfr, acc[pb][0], 0, 0, 0);
+        acc[pb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[t][1], bfr, acc[pb][1], 0, 0, 0);
+      }
+    }
+
+    if (y < H) {
+#pragma unroll
+      for (int pb = 0; pb < 4; ++pb) {
+        const int x = x0 + pb * 16 + (lane & 15);
+        if (x >= W) continue;
+        const bool valid = (y < ny) && (x < nx);
+        const int64_t po = (img + (int64_t)y * W + x) * kC;
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+          const int co = cb * 16 + 4 * (lane >> 4);
+          float v[4];
+          if constexpr (MODE == 0) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = leaky(acc[pb][cb][e] + bias_r[cb][e]);
+          } else {
+            const uint2 g2 = *reinterpret_cast<const uint2*>(aux + po + co);
+            const bf16* g = reinterpret_cast<const bf16*>(&g2);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = acc[pb][cb][e] * (bf2f(g[e]) > 0.f ? 1.f : 0.01f);
+          }
+          bf16 h[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) h[e] = valid ? f2bf(v[e]) : f2bf(0.f);
+          *reinterpret_cast<uint2*>(out + po + co) = *reinterpret_cast<const uint2*>(h);
+        }
+      }
     }
   }
+}
+
+// persistent grid for the 3x3 kernels: enough blocks to fill the chip a few
+// times over, each walking a contiguous run of tiles
+inline void conv_grid(int B, int H, int W, int* blocks, int* per) {
+  const int64_t nty = (H + kTileH - 1) / kTileH, ntx = (W + kTileW - 1) / kTileW;
+  const int64_t ntiles = (int64_t)B * nty * ntx;
+  constexpr int64_t kTarget = 2048;
+  const int64_t p = (ntiles + kTarget - 1) / kTarget;
+  *per = (int)(p < 1 ? 1 : p);
+  *blocks = (int)((ntiles + *per - 1) / *per);
 }
 
 // ------------------------------------------------------- 1x1 head (MFMA) --
@@ -273,6 +310,15 @@ head_bwd_kernel(const float* __restrict__ dY, const bf16* __restrict__ a16,
   const int yy = (int)(p / W), xx = (int)(p % W);
   const bool valid = yy < ny && xx < nx;
   const int64_t q = (int64_t)b * npx + p;
+  if (!valid) {  // outside the extent: zero gradients, no arithmetic
+    const uint4 z = {0, 0, 0, 0};
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) {
+      reinterpret_cast<uint4*>(dz16 + q * kC)[c4] = z;
+      reinterpret_cast<uint4*>(dYb + q * kC)[c4] = z;
+    }
+    return;
+  }
   float d[kC];
 #pragma unroll
   for (int c = 0; c < kC; ++c) d[c] = (c < cout && valid) ? dY[q * cout + c] : 0.f;
@@ -300,26 +346,33 @@ head_bwd_kernel(const float* __restrict__ dY, const bf16* __restrict__ a16,
 // ------------------------------------------------------------ wgrad MFMA --
 // dW[tap][co][ci] = sum_p A[p + delta_tap][ci] * dZ[p][co],  db[co] = sum_p dZ[p][co].
 // Persistent blocks over tiles of 1 row x 64 pixels; TAPS = 9 (3x3) or 1 (1x1).
-// Output blocks (tap, co-block, ci-block) of 16x16 are split over the 4 waves.
+// Wave w owns the output blocks (co-block w&1, ci-block w>>1) of all taps.
+// LDS holds dZ and the activation rows transposed ([channel][pixel]), one
+// copy each, written as packed 2-pixel dwords.  The x-shift s of a tap is
+// made in registers from two aligned 16-byte reads: s=0 is the first read,
+// s=2 a dword select, s=1 a 16-bit funnel shift (v_alignbit).
 // partials: [gridDim.x][TAPS*1024 + 32] fp32.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
 template <int TAPS>
 __global__ void __launch_bounds__(256)
 wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
                   const int32_t* __restrict__ ext, int B, int H, int W, float* __restrict__ partials) {
   constexpr int ROWS = TAPS == 9 ? 3 : 1;
-  constexpr int SHIFTS = TAPS == 9 ? 3 : 1;
-  constexpr int NBLK = TAPS * 4;
-  constexpr int PER_WAVE = NBLK / 4;
-  __shared__ __attribute__((aligned(16))) bf16 lds_dz[kC][kChStride];
-  __shared__ __attribute__((aligned(16))) bf16 lds_act[ROWS * SHIFTS][kC][kChStride];
+  constexpr int NT = TAPS;  // accumulators per wave
+  __shared__ __attribute__((aligned(16))) uint32_t lds_dz[kC][kChStride / 2];
+  __shared__ __attribute__((aligned(16))) uint32_t lds_act[ROWS][kC][kChStride / 2];
   __shared__ float dbred[256][8];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int cb = wave & 1, cib = wave >> 1;
   const int ntx = (W + kTileW - 1) / kTileW;
   const int64_t ntiles = (int64_t)B * H * ntx;
 
-  f32x4 acc[PER_WAVE];
+  f32x4 acc[NT];
 #pragma unroll
-  for (int i = 0; i < PER_WAVE; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < NT; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
   float dba[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) dba[e] = 0.f;
@@ -333,102 +386,126 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
     if (y >= ny || x0 >= nx) continue;  // block-uniform
     const int64_t img = (int64_t)b * H * W;
     __syncthreads();
-    {  // dZ row y, pixels x0 .. x0+63 -> lds_dz[ch][px]
-      const int px = tid >> 2, chunk = tid & 3;
-      const int x = x0 + px;
-      bf16x8 v;
-      if (x < nx) v = *reinterpret_cast<const bf16x8*>(dz + (img + (int64_t)y * W + x) * kC + chunk * 8);
-      else {
+    if (tid < 128) {  // dZ row y, pixel pairs (x0 + 2q, x0 + 2q + 1) -> lds_dz[ch][q]
+      const int q = tid >> 2, chunk = tid & 3;
+      uint4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
+      const int xa = x0 + 2 * q;
+      if (xa < nx) v0 = *reinterpret_cast<const uint4*>(dz + (img + (int64_t)y * W + xa) * kC + chunk * 8);
+      if (xa + 1 < nx) v1 = *reinterpret_cast<const uint4*>(dz + (img + (int64_t)y * W + xa + 1) * kC + chunk * 8);
+      const uint32_t a[4] = {v0.x, v0.y, v0.z, v0.w}, c[4] = {v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = f2bf(0.f);
-      }
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        lds_dz[chunk * 8 + e][px] = v[e];
-        dba[e] += bf2f(v[e]);
+      for (int i = 0; i < 4; ++i) {
+        // channel 2i: low halves; channel 2i+1: high halves (bf16 bit patterns)
+        lds_dz[chunk * 8 + 2 * i][q] = (a[i] & 0xFFFFu) | (c[i] << 16);
+        lds_dz[chunk * 8 + 2 * i + 1][q] = (a[i] >> 16) | (c[i] & 0xFFFF0000u);
+        dba[2 * i] += __uint_as_float(a[i] << 16) + __uint_as_float(c[i] << 16);
+        dba[2 * i + 1] += __uint_as_float(a[i] & 0xFFFF0000u) + __uint_as_float(c[i] & 0xFFFF0000u);
       }
     }
-    // activation rows (y-1 .. y+1 for 3x3), pixels x0-1 .. x0+64, x-shifted copies
-    constexpr int kChunks = ROWS * (kTileW + 2) * 4;
-    for (int c = tid; c < kChunks; c += 256) {
+    // activation rows (y-1 .. y+1 for 3x3), pixels x0-1+t, t = 0..65, as pairs
+    constexpr int kPairs = (kTileW + 2) / 2;
+    for (int c = tid; c < ROWS * kPairs * 4; c += 256) {
       const int chunk = c & 3;
-      const int t = (c >> 2) % (kTileW + 2);
-      const int row = (c >> 2) / (kTileW + 2);
+      const int q = (c >> 2) % kPairs;
+      const int row = (c >> 2) / kPairs;
       const int yy = (ROWS == 3) ? y - 1 + row : y;
-      const int xx = x0 - 1 + t;
-      bf16x8 v;
-      if (yy >= 0 && yy < ny && xx >= 0 && xx < nx)
-        v = *reinterpret_cast<const bf16x8*>(act + (img + (int64_t)yy * W + xx) * kC + chunk * 8);
-      else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = f2bf(0.f);
+      const int xa = x0 - 1 + 2 * q;
+      uint4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
+      if (yy >= 0 && yy < ny) {
+        if (xa >= 0 && xa < nx) v0 = *reinterpret_cast<const uint4*>(act + (img + (int64_t)yy * W + xa) * kC + chunk * 8);
+        if (xa + 1 >= 0 && xa + 1 < nx) v1 = *reinterpret_cast<const uint4*>(act + (img + (int64_t)yy * W + xa + 1) * kC + chunk * 8);
       }
+      const uint32_t a[4] = {v0.x, v0.y, v0.z, v0.w}, c2[4] = {v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
-      for (int s = 0; s < SHIFTS; ++s) {
-        const int p = t - ((SHIFTS == 3) ? s : 1);
-        if (p >= 0 && p < kTileW) {
-#pragma unroll
-          for (int e = 0; e < 8; ++e) lds_act[row * SHIFTS + s][chunk * 8 + e][p] = v[e];
-        }
+      for (int i = 0; i < 4; ++i) {
+        lds_act[row][chunk * 8 + 2 * i][q] = (a[i] & 0xFFFFu) | (c2[i] << 16);
+        lds_act[row][chunk * 8 + 2 * i + 1][q] = (a[i] >> 16) | (c2[i] & 0xFFFF0000u);
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < PER_WAVE; ++i) {
-      const int blk = wave * PER_WAVE + i;
-      const int tap = blk >> 2, cb = (blk >> 1) & 1, cib = blk & 1;
+    for (int ks = 0; ks < 2; ++ks) {
+      const int base = (ks * 32 + 8 * (lane >> 4)) / 2;  // dword index of the 8-pixel group
+      const u32x4 a4 = *reinterpret_cast<const u32x4*>(&lds_dz[cb * 16 + (lane & 15)][base]);
+      const bf16x8 afr = as_bf16x8(a4);
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&lds_dz[cb * 16 + (lane & 15)][ks * 32 + 8 * (lane >> 4)]);
-        const bf16x8 bb = *reinterpret_cast<const bf16x8*>(&lds_act[tap][cib * 16 + (lane & 15)][ks * 32 + 8 * (lane >> 4)]);
-        acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[i], 0, 0, 0);
+      for (int r = 0; r < ROWS; ++r) {
+        // activation pixels x0 - 1 + t with t = 2*base .. 2*base + 15
+        const uint32_t* src = &lds_act[r][cib * 16 + (lane & 15)][base];
+        const u32x4 c0 = *reinterpret_cast<const u32x4*>(src);
+        if constexpr (TAPS == 9) {
+          const u32x4 c1 = *reinterpret_cast<const u32x4*>(src + 4);
+          // s = 0: pixels x0 - 1 + p  -> t = p          (c0)
+          // s = 1: pixels x0 + p      -> t = p + 1      (16-bit funnel shift)
+          // s = 2: pixels x0 + 1 + p  -> t = p + 2      (dwords 1..4)
+          u32x4 s1, s2;
+          s1[0] = __builtin_amdgcn_alignbit(c0[1], c0[0], 16);
+          s1[1] = __builtin_amdgcn_alignbit(c0[2], c0[1], 16);
+          s1[2] = __builtin_amdgcn_alignbit(c0[3], c0[2], 16);
+          s1[3] = __builtin_amdgcn_alignbit(c1[0], c0[3], 16);
+          s2[0] = c0[1]; s2[1] = c0[2]; s2[2] = c0[3]; s2[3] = c1[0];
+          acc[r * 3 + 0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(c0), acc[r * 3 + 0], 0, 0, 0);
+          acc[r * 3 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(s1), acc[r * 3 + 1], 0, 0, 0);
+          acc[r * 3 + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(s2), acc[r * 3 + 2], 0, 0, 0);
+        } else {
+          // 1x1: pixel x0 + p -> t = p + 1
+          const u32x4 c1 = *reinterpret_cast<const u32x4*>(src + 4);
+          u32x4 s1;
+          s1[0] = __builtin_amdgcn_alignbit(c0[1], c0[0], 16);
+          s1[1] = __builtin_amdgcn_alignbit(c0[2], c0[1], 16);
+          s1[2] = __builtin_amdgcn_alignbit(c0[3], c0[2], 16);
+          s1[3] = __builtin_amdgcn_alignbit(c1[0], c0[3], 16);
+          acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(s1), acc[0], 0, 0, 0);
+        }
       }
     }
   }
   // partial dW: D[co][ci], lane: ci = cib*16 + lane%16, co = cb*16 + 4*(lane/16) + e
   float* P = partials + (int64_t)blockIdx.x * (TAPS * kC * kC + kC);
+  const int ci = cib * 16 + (lane & 15);
 #pragma unroll
-  for (int i = 0; i < PER_WAVE; ++i) {
-    const int blk = wave * PER_WAVE + i;
-    const int tap = blk >> 2, cb = (blk >> 1) & 1, cib = blk & 1;
-    const int ci = cib * 16 + (lane & 15);
+  for (int tap = 0; tap < NT; ++tap)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int co = cb * 16 + 4 * (lane >> 4) + e;
-      P[(tap * kC + co) * kC + ci] = acc[i][e];
+      P[(tap * kC + co) * kC + ci] = acc[tap][e];
     }
-  }
-  // partial db: threads with the same (tid & 3) hold channels (tid&3)*8 .. +8
+  // partial db: threads tid < 128 hold channels (tid&3)*8 .. +8
 #pragma unroll
   for (int e = 0; e < 8; ++e) dbred[tid][e] = dba[e];
   __syncthreads();
   if (tid < kC) {
     const int chunk = tid >> 3, e = tid & 7;
     float s = 0.f;
-    for (int t = chunk; t < 256; t += 4) s += dbred[t][e];
+    for (int t = chunk; t < 128; t += 4) s += dbred[t][e];
     P[TAPS * kC * kC + tid] = s;
   }
 }
 
 // out_hwio[tap][ci][co] = sum_blk P[blk][(tap*32 + co)*32 + ci] (co < cout);
-// db[co] = sum_blk P[blk][TAPS*1024 + co].
+// db[co] = sum_blk P[blk][TAPS*1024 + co].  A block reduces 64 consecutive
+// partial entries: thread (g, j) sums blocks g, g+4, ... of entry j, then the
+// four group sums are added in a fixed order (deterministic).
 __global__ void __launch_bounds__(256)
 wgrad_reduce_kernel(const float* __restrict__ P, int nblk, int taps, int cout,
                     float* __restrict__ dW, float* __restrict__ db) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nw = taps * kC * cout;
-  const int64_t stride = (int64_t)taps * kC * kC + kC;
-  if (i < nw) {
-    const int co = i % cout, ci = (i / cout) % kC, tap = i / (cout * kC);
-    const int64_t src = (tap * kC + co) * kC + ci;
-    float s = 0.f;
-    for (int k = 0; k < nblk; ++k) s += P[k * stride + src];
-    dW[i] = s;
-  } else if (i < nw + cout) {
-    const int co = i - nw;
-    float s = 0.f;
-    for (int k = 0; k < nblk; ++k) s += P[k * stride + (int64_t)taps * kC * kC + co];
-    db[co] = s;
+  __shared__ float part[4][64];
+  const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int stride = taps * kC * kC + kC;
+  const int src = blockIdx.x * 64 + j;  // partial-entry index
+  float s = 0.f;
+  if (src < stride)
+    for (int k = g; k < nblk; k += 4) s += P[(int64_t)k * stride + src];
+  part[g][j] = s;
+  __syncthreads();
+  if (g != 0 || src >= stride) return;
+  const float t = ((part[0][j] + part[1][j]) + part[2][j]) + part[3][j];
+  if (src < taps * kC * kC) {
+    const int ci = src % kC, co = (src / kC) % kC, tap = src / (kC * kC);
+    if (co < cout) dW[(tap * kC + ci) * cout + co] = t;
+  } else {
+    const int co = src - taps * kC * kC;
+    if (co < cout) db[co] = t;
   }
 }
 
@@ -503,14 +580,19 @@ conv_in_wgrad_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
   if (g == 0) P[9 * kCin * kC + co] = dbacc;
 }
 
+// out[i] = sum_blk P[blk][i], i < count (same 2-D deterministic scheme)
 __global__ void __launch_bounds__(256)
 sum_partials_kernel(const float* __restrict__ P, int nblk, int stride, int count,
                     float* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+  __shared__ float part[4][64];
+  const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + j;
   float s = 0.f;
-  for (int k = 0; k < nblk; ++k) s += P[(int64_t)k * stride + i];
-  out[i] = s;
+  if (i < count)
+    for (int k = g; k < nblk; k += 4) s += P[(int64_t)k * stride + i];
+  part[g][j] = s;
+  __syncthreads();
+  if (g == 0 && i < count) out[i] = ((part[0][j] + part[1][j]) + part[2][j]) + part[3][j];
 }
 
 // ------------------------------------------------------------ optimizer --
@@ -611,10 +693,11 @@ int igm_conv3x3_fwd_bf16(const void* in, const void* wpacked, const float* bias,
   IGM_CHECK_ARG(in && wpacked && bias && out, "null pointer");
   IGM_CHECK_ARG(in != out, "in and out must differ");
   IGM_CHECK_ARG(B >= 1 && H >= 1 && W >= 1 && B <= 65535, "bad sizes");
-  dim3 grid((W + kTileW - 1) / kTileW, (H + kTileH - 1) / kTileH, B);
-  hipLaunchKernelGGL(conv3x3_mfma_kernel<0>, grid, dim3(256), 0, (hipStream_t)stream,
-                     (const bf16*)in, (const bf16*)wpacked, bias, (const bf16*)nullptr, extents, H, W,
-                     (bf16*)out);
+  int blocks, per;
+  conv_grid(B, H, W, &blocks, &per);
+  hipLaunchKernelGGL(conv3x3_mfma_kernel<0>, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                     (const bf16*)in, (const bf16*)wpacked, bias, (const bf16*)nullptr, extents, B, H, W,
+                     per, (bf16*)out);
   IGM_CHECK_LAUNCH();
   return 0;
 }
@@ -625,10 +708,11 @@ int igm_conv3x3_dgrad_bf16(const void* dz, const void* wpacked_dgrad, const void
   IGM_CHECK_ARG(dz && wpacked_dgrad && act_below && dz_below, "null pointer");
   IGM_CHECK_ARG(dz != dz_below, "in and out must differ");
   IGM_CHECK_ARG(B >= 1 && H >= 1 && W >= 1 && B <= 65535, "bad sizes");
-  dim3 grid((W + kTileW - 1) / kTileW, (H + kTileH - 1) / kTileH, B);
-  hipLaunchKernelGGL(conv3x3_mfma_kernel<1>, grid, dim3(256), 0, (hipStream_t)stream,
+  int blocks, per;
+  conv_grid(B, H, W, &blocks, &per);
+  hipLaunchKernelGGL(conv3x3_mfma_kernel<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
                      (const bf16*)dz, (const bf16*)wpacked_dgrad, (const float*)nullptr,
-                     (const bf16*)act_below, extents, H, W, (bf16*)dz_below);
+                     (const bf16*)act_below, extents, B, H, W, per, (bf16*)dz_below);
   IGM_CHECK_LAUNCH();
   return 0;
 }
@@ -683,9 +767,9 @@ int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int
     hipLaunchKernelGGL(wgrad_mfma_kernel<1>, dim3(nblocks), dim3(256), 0, st, (const bf16*)act,
                        (const bf16*)dz, extents, B, H, W, workspace);
   IGM_CHECK_LAUNCH();
-  const int n = taps * kC * cout + cout;
-  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(grid_for(n)), dim3(256), 0, st, workspace, nblocks,
-                     taps, cout, dW_hwio, db);
+  const int entries = taps * kC * kC + kC;
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((entries + 63) / 64), dim3(256), 0, st, workspace,
+                     nblocks, taps, cout, dW_hwio, db);
   IGM_CHECK_LAUNCH();
   return 0;
 }
@@ -702,7 +786,7 @@ int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const f
   IGM_CHECK_LAUNCH();
   const int n = 9 * kCin * kC + kC;
   // the layer-1 partial layout is [blk][1440 + 32] = [blk][n]: a plain column sum
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(grid_for(9 * kCin * kC)), dim3(256), 0, st,
+  hipLaunchKernelGGL(sum_partials_kernel, dim3((9 * kCin * kC + 63) / 64), dim3(256), 0, st,
                      workspace, nblocks, n, 9 * kCin * kC, dW_hwio);
   IGM_CHECK_LAUNCH();
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, workspace + 9 * kCin * kC,

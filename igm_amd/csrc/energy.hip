@@ -242,6 +242,23 @@ energy_pass_kernel(const PassArgs<T> a) {
       const T bx = bload(rC, co + 5 * ps, T()), by = bload(rC, co + 6 * ps, T());
       const bool emit_row = own && (jc >= j0);
       const uint32_t on = node_off(jc, emit_row);  // output node (jc, ic), masked
+      // Adam moments of the output node row, issued before the level sweep so
+      // their latency hides behind it (masked lanes read 0)
+      // (float32 only: the float64 pass keeps its loads at the point of use,
+      // where its register budget stays within the VGPR file without SGPR spills)
+      constexpr bool kPrefetchMoments = (MODE == 2) && (sizeof(T) == 4);
+      T mu_[NZ], vu_[NZ], mv_[NZ], vv_[NZ];
+      if constexpr (kPrefetchMoments) {
+        const rsrc_t rmu = make_rsrc(a.mU, a.vel_bytes), rvu = make_rsrc(a.vU, a.vel_bytes);
+        const rsrc_t rmv = make_rsrc(a.mV, a.vel_bytes), rvv = make_rsrc(a.vV, a.vel_bytes);
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          mu_[k] = bload(rmu, on + k * kstep, T());
+          vu_[k] = bload(rvu, on + k * kstep, T());
+          mv_[k] = bload(rmv, on + k * kstep, T());
+          vv_[k] = bload(rvv, on + k * kstep, T());
+        }
+      }
 
       // scatter of the adjoint of level k to the 4 corner nodes, then the
       // output of node (jc, ic), level k (masked stores)
@@ -256,7 +273,9 @@ energy_pass_kernel(const PassArgs<T> a) {
             bstore(g, make_rsrc(a.gU, a.vel_bytes), ok);
           } else if constexpr (MODE == 2) {
             const rsrc_t rm = make_rsrc(a.mU, a.vel_bytes), rv = make_rsrc(a.vU, a.vel_bytes);
-            T m = bload(rm, ok, T()), v = bload(rv, ok, T());
+            T m, v;
+            if constexpr (kPrefetchMoments) { m = mu_[k]; v = vu_[k]; }
+            else { m = bload(rm, ok, T()); v = bload(rv, ok, T()); }
             m += (g - m) * a.omb1;
             v += (g * g - v) * a.omb2;
             bstore(up[k] - (m * a.alpha) * rcp(sqrt_(v) + a.eps), make_rsrc(a.Uo, a.vel_bytes), ok);
@@ -273,7 +292,9 @@ energy_pass_kernel(const PassArgs<T> a) {
             bstore(g, make_rsrc(a.gV, a.vel_bytes), ok);
           } else if constexpr (MODE == 2) {
             const rsrc_t rm = make_rsrc(a.mV, a.vel_bytes), rv = make_rsrc(a.vV, a.vel_bytes);
-            T m = bload(rm, ok, T()), v = bload(rv, ok, T());
+            T m, v;
+            if constexpr (kPrefetchMoments) { m = mv_[k]; v = vv_[k]; }
+            else { m = bload(rm, ok, T()); v = bload(rv, ok, T()); }
             m += (g - m) * a.omb1;
             v += (g * g - v) * a.omb2;
             bstore(vp[k] - (m * a.alpha) * rcp(sqrt_(v) + a.eps), make_rsrc(a.Vo, a.vel_bytes), ok);

@@ -134,7 +134,7 @@ class CNN:
         return (Y, acts) if keep else Y
 
     # ----------------------------------------------------------- backward --
-    def backward(self, dY, acts, fields, extents=None, nblocks=1024):
+    def backward(self, dY, acts, fields, extents=None, nblocks=512):
         """Gradient of sum(dY * Y) w.r.t. all parameters, into self.g."""
         fptrs, (fb, fr, fc), B, H, W = fields
         ext = L.ptr(extents)

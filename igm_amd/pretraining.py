@@ -77,11 +77,12 @@ class Batcher:
         H = max(p.shape[0] for p in parts)
         W = max(p.shape[1] for p in parts)
         X = torch.zeros((self.batch, H, W, 5), dtype=torch.float32, device=self.dev)
-        ext = []
+        ext = torch.empty((self.batch, 2), dtype=torch.int32, pin_memory=True)
         for b, p in enumerate(parts):
             X[b, :p.shape[0], :p.shape[1]] = p
-            ext.append((p.shape[0], p.shape[1]))
-        return X, torch.tensor(ext, dtype=torch.int32, device=self.dev)
+            ext[b, 0], ext[b, 1] = p.shape[0], p.shape[1]
+        # pinned host buffer + non-blocking copy: no host/device synchronisation per step
+        return X, ext.to(self.dev, non_blocking=True)
 
 
 def train(cnn, pool, steps, dzeta, phys, batch=32, lr=1e-4, lr_end=None, clipnorm=1.0, epsilon=1e-7,

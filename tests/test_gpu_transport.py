@@ -56,7 +56,12 @@ def test_time_and_smb_processes(dev):
     ref = T.smb_simple(st.usurf.cpu().numpy(), 0.005, 0.005, 2800.0, 2.0)
     np.testing.assert_allclose(st.smb.cpu().numpy(), ref, rtol=1e-6, atol=1e-6)
     timemod.update(cfg, st)
-    assert st.dt == pytest.approx(T.cfl_dt(st.ubar.cpu().numpy(), st.vbar.cpu().numpy(), 100.0, 0.3, 1.0), rel=1e-6)
+    # time.py:54-59: the first step snaps to the first save time (t = start), dt = 0
+    assert st.dt == 0.0 and st.saveresult
+    assert st.dt_target == pytest.approx(T.cfl_dt(st.ubar.cpu().numpy(), st.vbar.cpu().numpy(), 100.0, 0.3, 1.0),
+                                         rel=1e-6)
+    timemod.update(cfg, st)
+    assert st.dt == pytest.approx(st.dt_target) and st.t == pytest.approx(2000.0 + st.dt_target)
     vol0 = float(st.thk.double().sum())
     thkmod.update(cfg, st)
     assert torch.isfinite(st.thk).all() and float(st.thk.min()) >= 0.0
