@@ -3,18 +3,22 @@
 
 One step = one velocity field of the emulated ice-flow update on the
 synthetic 20-dome glacier (1000 x 1000, dx = 50 m, Nz = 10): CNN inference on
-the MFMA kernels, thk > 0 mask + derived fields, and the energy-gap check of
-the fallback (float64-reduced energy; fallback solve when the gap exceeds
-1 %).  Inputs are resident in HBM before the timed region.
+the MFMA kernels, then the shortest warm-started polish by the float32
+energy solver (schedule searched here, before the timed region) that brings
+the field to <= 1 % relative L2 error against the float64 minimiser, the
+thk > 0 mask and derived fields, and the float64-reduced energy of the
+fallback check.  Inputs are resident in HBM before the timed region.
 
 Reported next to it:
-* the float32 energy solver (the reference's plain method: legacy Adam,
-  lr = 1, cold start) timed to <= 1 % rel L2 error vs the float64 minimiser;
-* the float64 minimiser itself (float64 GPU solve, decaying step);
+* CNN inference alone (fields/s and its error);
+* the float32 energy solver (the reference's plain method, legacy Adam)
+  from a cold start, with the same schedule family and search, timed to
+  <= 1 %, and the reference's own settings (lr = 1, no decay) as a curve;
+* the float64 minimiser itself (float32 + float64 GPU solve, decaying step);
 * live rooflines (HIP events on the launch stream) of the dominant kernel
   (the 3x3 bf16 MFMA conv) and of the energy/adjoint kernel;
 * a CPU baseline: the oracle (torch-CPU float32 restatement) CNN inference
-  and solver iteration on a bounded sample, on this host's cores.
+  on this host's cores.
 """
 import argparse
 import json
@@ -32,17 +36,14 @@ METRIC = "velocity fields/s at 1000²×Nz10: emulator vs energy solver at ≤1% 
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 BENCH_WEIGHTS = ("igm_amd_cnn_16_32_nz10.safetensors", "pinnbp_10_4_cnn_16_32_2_1_a.safetensors")
+# schedule family searched for both paths: initial step, then total iterations
+LR0S = (1.0, 3.0, 10.0)
+REFINE_TOTALS = (100, 200, 300, 500, 700, 1000, 1500, 2000, 3000, 4500, 7000, 10000, 14000)
+SOLVER_TOTALS = (1000, 1500, 2000, 3000, 4500, 7000, 10000, 14000, 20000)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def rel_err(U, V, Ur, Vr, mask):
-    d = torch.stack([U, V]).double() - torch.stack([Ur, Vr]).double()
-    r = torch.stack([Ur, Vr]).double()
-    m = mask.to(torch.float64)
-    return float(torch.linalg.vector_norm(d * m) / torch.linalg.vector_norm(r * m))
 
 
 class Timer:
@@ -62,29 +63,18 @@ class Timer:
         self.ms = self.a.elapsed_time(self.b)
 
 
-def adam_schedule(lr0, n_flat, n_decay, lr_end=1e-3):
-    """Step schedule of the tuned solver: lr0 for n_flat iterations, then a
-    geometric decay to lr_end over n_decay iterations."""
-    d = (lr_end / lr0) ** (1.0 / max(1, n_decay))
-    return lambda i: lr0 if i < n_flat else max(lr_end, lr0 * d ** (i - n_flat))
-
-
 def reference_solution(f32, f64, thk, nz, dzeta, n_flat=8000, n_decay=20000, n_refine=2000):
-    """float64 minimiser: float32 Adam, lr 10 then a geometric decay to 1e-3
-    (converges on this workload: runs of 14k and 28k iterations agree to 3e-3),
-    then float64 Adam from that point with the step decaying 1e-3 -> 1e-5."""
+    """float64 minimiser of the workload (converges here: runs of 14k and 28k
+    float32 iterations agree to 3e-3, and the float64 stage moves the field by
+    ~1e-8)."""
     from igm_amd.energy import EnergyOperator
-    from igm_amd.solver import LegacyAdamState, solve
+    from igm_amd.solver import converged_solution
     from oracle.energy import DEFAULT_PHYSICS
     op32 = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f32)
-    z = torch.zeros((nz,) + tuple(thk.shape), dtype=torch.float32, device=thk.device)
-    r = solve(op32, z, z.clone(), n_flat + n_decay, LegacyAdamState(lr=1.0), early_stopping=False,
-              lr_schedule=adam_schedule(10.0, n_flat, n_decay))
     op64 = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f64, dtype=torch.float64)
-    r2 = solve(op64, r.U.double(), r.V.double(), n_refine, LegacyAdamState(lr=1.0), early_stopping=False,
-               lr_schedule=adam_schedule(1e-3, 0, n_refine, lr_end=1e-5))
-    change = rel_err(r.U, r.V, r2.U, r2.V, thk > 0)
-    return r2.U, r2.V, float(r2.costs[-1]), change, op64
+    z = torch.zeros((nz,) + tuple(thk.shape), dtype=torch.float32, device=thk.device)
+    U, V, J, change = converged_solution(op32, op64, z, z.clone(), n_flat, n_decay, n_refine)
+    return U, V, J, change, op64
 
 
 def main():
@@ -108,7 +98,7 @@ def main():
     from igm_amd import _lib as L
     from igm_amd.energy import EnergyOperator
     from igm_amd.fields import velocity_outputs
-    from igm_amd.solver import LegacyAdamState, solve
+    from igm_amd.solver import LegacyAdamState, adam_schedule, rel_err, solve
     from oracle.energy import DEFAULT_PHYSICS
 
     dev = torch.device("cuda:0")
@@ -158,9 +148,50 @@ def main():
     log(f"[bench] float64 reference: {args.ref_flat}+{args.ref_decay} its in {time.time() - t:.1f}s, "
         f"J={J64:.6f}, change under float64 refinement {ref_change:.2e}")
 
+    op32 = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f32)
+    z = torch.zeros((nz, Ny, Nx), dtype=torch.float32, device=dev)
+
+    def schedule_search(U0, V0, totals):
+        """Shortest schedule of the family {lr0 in LR0S; 30 % flat, then a
+        geometric decay to 1e-3} from (U0, V0) that ends <= 1 %: returns
+        (its, lr0, err) of the first total that does (or of the last tried)."""
+        best = None
+        for total in totals:
+            if total == 0:
+                cand = [(0, None, rel_err(U0, V0, Ur, Vr, mask))]
+            else:
+                cand = []
+                for lr0 in LR0S:
+                    n_flat = int(0.3 * total)
+                    r = solve(op32, U0, V0, total, LegacyAdamState(lr=lr0), early_stopping=False,
+                              lr_schedule=adam_schedule(lr0, n_flat, total - n_flat))
+                    cand.append((total, lr0, rel_err(r.U, r.V, Ur, Vr, mask)))
+            best = min(cand, key=lambda c: c[2])
+            log(f"[bench]   {total} its: best rel err {best[2]:.4f} (lr0 {best[1]})")
+            if best[2] <= 0.01:
+                break
+        return best
+
     # ------------------------------------------------- emulator path --
+    # CNN inference alone, then the shortest warm-started polish that brings
+    # the emulated field to <= 1 % (the timed emulated update runs both)
+    EM.update_iceflow_emulated(cfg, state)
+    U_emu, V_emu = state.U.clone(), state.V.clone()
+    inf_err = rel_err(U_emu, V_emu, Ur, Vr, mask)
+    with Timer() as tm:
+        for _ in range(args.steps):
+            EM.update_iceflow_emulated(cfg, state)
+    inf_ms = tm.ms / args.steps
+    log(f"[bench] emulator inference {inf_ms:.3f} ms, rel err {inf_err:.4f}; searching the polish")
+    n_ref, lr_ref, _ = schedule_search(U_emu, V_emu, (0,) + REFINE_TOTALS)
+    ice.emulator.refine.its = int(n_ref)
+    if lr_ref is not None:
+        ice.emulator.refine.lr = float(lr_ref)
+
     def emulator_step():
         EM.update_iceflow_emulated(cfg, state)
+        if ice.emulator.refine.its > 0:
+            iceflow.refine_emulated(cfg, state)
         if ice.emulator.fallback.enabled:
             iceflow.fallback_check(cfg, state)
 
@@ -175,19 +206,10 @@ def main():
     emu_ms = tm.ms / args.steps
     emu_fps = 1000.0 / emu_ms
     emu_err = rel_err(state.U, state.V, Ur, Vr, mask)
-    j_emu = getattr(state, "COST_EMULATED", float("nan"))
-    # inference only (no energy check)
-    with Timer() as tm:
-        for _ in range(args.steps):
-            EM.update_iceflow_emulated(cfg, state)
-    inf_ms = tm.ms / args.steps
-    inf_err = rel_err(state.U, state.V, Ur, Vr, mask)
-    log(f"[bench] emulator step {emu_ms:.3f} ms ({emu_fps:.1f} fields/s), err {emu_err:.4f}; "
-        f"inference only {inf_ms:.3f} ms, err {inf_err:.4f}; fallbacks {state.fallback_count}/{args.steps}")
+    log(f"[bench] emulated update (inference + {n_ref} polish its) {emu_ms:.3f} ms ({emu_fps:.1f} fields/s), "
+        f"rel err {emu_err:.4f}; fallbacks {state.fallback_count}/{args.steps}")
 
     # --------------------------------------------------- solver path --
-    op32 = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f32)
-    z = torch.zeros((nz, Ny, Nx), dtype=torch.float32, device=dev)
     # (a) the reference's own settings: Adam lr = step_size = 1, no decay
     U, V = z, z.clone()
     opt = LegacyAdamState(lr=1.0)
@@ -199,28 +221,19 @@ def main():
         if its >= args.solver_max:
             break
     ref_settings = {"lr": 1.0, "error_curve": curve}
-    # (b) the same method with a decaying step: the shortest schedule of the
-    # family lr0 = 10, 30 % flat + 70 % geometric decay to 1e-3 that ends <= 1 %
-    best = None
-    for total in (1500, 2500, 4000, 6000, 9000, 14000, 20000):
-        n_flat = int(0.3 * total)
-        sched = adam_schedule(10.0, n_flat, total - n_flat)
-        r = solve(op32, z, z.clone(), total, LegacyAdamState(lr=1.0), early_stopping=False, lr_schedule=sched)
-        e = rel_err(r.U, r.V, Ur, Vr, mask)
-        log(f"[bench] tuned solver schedule {total} its: rel err {e:.4f}")
-        if e <= 0.01:
-            best = (total, n_flat, e)
-            break
-    solver_reached = best is not None
-    n1, n_flat1, solver_err = best if best else (total, n_flat, e)
-    sched = adam_schedule(10.0, n_flat1, n1 - n_flat1)
+    # (b) the same method and schedule family as the polish, from zero
+    log("[bench] searching the solver schedule from zero")
+    n1, lr1, solver_err = schedule_search(z, z, SOLVER_TOTALS)
+    solver_reached = solver_err <= 0.01
+    n_flat1 = int(0.3 * n1)
+    sched = adam_schedule(lr1, n_flat1, n1 - n_flat1)
     solve(op32, z, z.clone(), 20, LegacyAdamState(lr=1.0), early_stopping=False)
     with Timer() as tm:
-        rs = solve(op32, z, z.clone(), n1, LegacyAdamState(lr=1.0), early_stopping=False, lr_schedule=sched)
+        rs = solve(op32, z, z.clone(), n1, LegacyAdamState(lr=lr1), early_stopping=False, lr_schedule=sched)
     solver_ms = tm.ms
     solver_fps = 1000.0 / solver_ms
     it_ms = solver_ms / n1
-    log(f"[bench] solver (tuned schedule): {n1} its to err {solver_err:.4f} (<= 1%: {solver_reached}) in "
+    log(f"[bench] solver: {n1} its (lr0 {lr1}) to err {solver_err:.4f} (<= 1%: {solver_reached}) in "
         f"{solver_ms:.1f} ms ({it_ms * 1e3:.1f} us/it) -> {solver_fps:.3f} fields/s; reference settings {curve}")
     # reference early-stopping behaviour (nbitmax large)
     res_es = solve(op32, z, z.clone(), 20000, LegacyAdamState(lr=1.0), early_stopping=True)
@@ -293,20 +306,22 @@ def main():
         "dtype": "bf16",
         "data": "synthetic",
         "config": {"workload": "domes1000: 1000x1000 grid, dx=50 m, Nz=10, 20 seeded domes on a random-field bed; "
-                               "emulated iceflow update (bf16 MFMA CNN 16x3x3x32 + 1x1, fp32 first layer) with "
-                               "float64 energy-gap fallback check vs the fp32 energy solver",
+                               "emulated iceflow update (bf16 MFMA CNN 16x3x3x32 + 1x1, fp32 first layer) "
+                               "polished by the shortest warm-started fp32 energy-solver schedule that reaches "
+                               "<= 1 % rel err vs the float64 minimiser, plus the float64 energy-gap check",
                    "weights": os.path.basename(wpath),
+                   "polish_its": int(n_ref), "polish_lr0": lr_ref,
                    "fallback_check_its": int(ice.emulator.fallback.check_its)},
         "roofline": {"bound": "mfma", "achieved": round(conv_tflops, 2), "peak": PEAK_BF16_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(conv_tflops / PEAK_BF16_TFLOPS, 4), "traffic": None,
                      "kernel": "conv3x3_mfma_kernel<0> (32->32, 1000x1000)", "us_per_launch": round(conv_us, 2)},
         "cpu_baseline": cpu,
-        "emulator_rel_err_vs_fp64": round(emu_err, 5),
-        "emulator_inference_ms": round(inf_ms, 4),
-        "emulator_inference_rel_err": round(inf_err, 5),
+        "emulated_rel_err_vs_fp64": round(emu_err, 5),
+        "emulator_inference": {"ms": round(inf_ms, 4), "fields_per_s": round(1000.0 / inf_ms, 1),
+                               "rel_err_vs_fp64": round(inf_err, 5)},
         "fallbacks": int(state.fallback_count),
         "solver_fields_per_s": round(solver_fps, 4),
-        "solver_schedule": {"lr0": 10.0, "flat": int(n_flat1), "decay_to_1e-3": int(n1 - n_flat1),
+        "solver_schedule": {"lr0": lr1, "flat": int(n_flat1), "decay_to_1e-3": int(n1 - n_flat1),
                             "rel_err": round(solver_err, 5)},
         "solver_iters_to_1pct": int(n1),
         "solver_reached_1pct": bool(solver_reached),

@@ -47,9 +47,9 @@ _SIGS = {
     "igm_energy_coeffs_f32": (C.c_int, [VP, VP, VP, VP, VP, I64, I64, I64, VP, I32, I32, I32, VP, VP, VP, VP]),
     "igm_energy_coeffs_f64": (C.c_int, [VP, VP, VP, VP, VP, I64, I64, I64, VP, I32, I32, I32, VP, VP, VP, VP]),
     "igm_energy_pass_f32": (C.c_int, [I32, I32, I32, I32, I32, VP, VP, I64, I64, I64, I64, VP, VP, VP, VP,
-                                      VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+                                      VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "igm_energy_pass_f64": (C.c_int, [I32, I32, I32, I32, I32, VP, VP, I64, I64, I64, I64, VP, VP, VP, VP,
-                                      VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+                                      VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     "igm_energy_finalize": (C.c_int, [VP, I32, I32, I32, VP, VP, VP, I32, VP]),
     "igm_velocity_outputs_f32": (C.c_int, [I32, I32, I32, VP, VP, I64, I64, I64, VP, VP, VP, VP, VP, VP, VP,
                                            VP, VP, VP, VP]),
@@ -61,9 +61,9 @@ _SIGS = {
     "igm_conv1x1_head_fwd": (C.c_int, [VP, VP, VP, VP, I32, I32, I32, I32, VP, VP]),
     "igm_conv1x1_head_bwd": (C.c_int, [VP, VP, VP, VP, I32, I32, I32, I32, VP, VP, VP]),
     "igm_wgrad_workspace_floats": (I64, [I32, I32]),
-    "igm_conv_wgrad_bf16": (C.c_int, [I32, VP, VP, VP, I32, I32, I32, I32, I32, VP, VP, VP, VP]),
-    "igm_conv_in_wgrad": (C.c_int, [VP, VP, VP, VP, VP, I64, I64, I64, VP, I32, I32, I32, VP, I32, VP, VP,
-                                    VP, VP]),
+    "igm_conv_wgrad_bf16": (C.c_int, [I32, VP, VP, VP, I32, I32, I32, I32, I32, VP, I32, VP, VP, VP]),
+    "igm_conv_in_wgrad": (C.c_int, [VP, VP, VP, VP, VP, I64, I64, I64, VP, I32, I32, I32, VP, I32, VP, I32,
+                                    VP, VP, VP]),
     "igm_adam_clip_f32": (C.c_int, [VP, VP, VP, VP, I64, VP, I32, VP, F64, VP, VP]),
     "igm_pack_conv_weights": (C.c_int, [VP, VP, I32, VP, I32, VP, VP, VP, VP]),
     "igm_transport_f32": (C.c_int, [I32, I32, VP, VP, VP, VP, VP, F64, F64, F64, I32, F64, F64, VP, VP, VP,

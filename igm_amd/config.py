@@ -6,7 +6,8 @@ smb_simple.yaml, as an attribute-access tree (any object with the same
 attributes -- an OmegaConf DictConfig for instance -- works the same).
 
 Keys this build adds (absent from the reference) live under
-``processes.iceflow.emulator.fallback`` and ``processes.iceflow.solver``
+``processes.iceflow.emulator.fallback``, ``processes.iceflow.emulator.refine``
+and ``processes.iceflow.solver``
 (``check_every``); each is documented where it is read.
 """
 import copy
@@ -94,6 +95,10 @@ ICEFLOW = {
             "check_freq": 1,          # check every check_freq update() calls
             "max_its": 2000,          # solver iterations of a fallback solve
         },
+        # build-specific polish of every emulated field: `its` plain-solver
+        # iterations warm-started from it, step lr for flat_frac of them,
+        # then a geometric decay to lr_end (0 its: off)
+        "refine": {"its": 0, "lr": 3.0, "flat_frac": 0.3, "lr_end": 1.0e-3},
     },
 }
 

@@ -487,7 +487,7 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
 // partial entries: thread (g, j) sums blocks g, g+4, ... of entry j, then the
 // four group sums are added in a fixed order (deterministic).
 __global__ void __launch_bounds__(256)
-wgrad_reduce_kernel(const float* __restrict__ P, int nblk, int taps, int cout,
+wgrad_reduce_kernel(const float* __restrict__ P, int nblk, int taps, int cout, int accumulate,
                     float* __restrict__ dW, float* __restrict__ db) {
   __shared__ float part[4][64];
   const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
@@ -502,10 +502,13 @@ wgrad_reduce_kernel(const float* __restrict__ P, int nblk, int taps, int cout,
   const float t = ((part[0][j] + part[1][j]) + part[2][j]) + part[3][j];
   if (src < taps * kC * kC) {
     const int ci = src % kC, co = (src / kC) % kC, tap = src / (kC * kC);
-    if (co < cout) dW[(tap * kC + ci) * cout + co] = t;
+    if (co < cout) {
+      float* d = dW + (tap * kC + ci) * cout + co;
+      *d = accumulate ? *d + t : t;
+    }
   } else {
     const int co = src - taps * kC * kC;
-    if (co < cout) db[co] = t;
+    if (co < cout) db[co] = accumulate ? db[co] + t : t;
   }
 }
 
@@ -582,7 +585,7 @@ conv_in_wgrad_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
 
 // out[i] = sum_blk P[blk][i], i < count (same 2-D deterministic scheme)
 __global__ void __launch_bounds__(256)
-sum_partials_kernel(const float* __restrict__ P, int nblk, int stride, int count,
+sum_partials_kernel(const float* __restrict__ P, int nblk, int stride, int count, int accumulate,
                     float* __restrict__ out) {
   __shared__ float part[4][64];
   const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
@@ -592,7 +595,10 @@ sum_partials_kernel(const float* __restrict__ P, int nblk, int stride, int count
     for (int k = g; k < nblk; k += 4) s += P[(int64_t)k * stride + i];
   part[g][j] = s;
   __syncthreads();
-  if (g == 0 && i < count) out[i] = ((part[0][j] + part[1][j]) + part[2][j]) + part[3][j];
+  if (g == 0 && i < count) {
+    const float t = ((part[0][j] + part[1][j]) + part[2][j]) + part[3][j];
+    out[i] = accumulate ? out[i] + t : t;
+  }
 }
 
 // ------------------------------------------------------------ optimizer --
@@ -754,7 +760,8 @@ int64_t igm_wgrad_workspace_floats(int32_t taps, int32_t nblocks) {
 
 int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int32_t* extents,
                         int32_t B, int32_t H, int32_t W, int32_t cout, int32_t nblocks,
-                        float* workspace, float* dW_hwio, float* db, void* stream) {
+                        float* workspace, int32_t accumulate, float* dW_hwio, float* db,
+                        void* stream) {
   IGM_CHECK_ARG(act && dz && workspace && dW_hwio && db, "null pointer");
   IGM_CHECK_ARG(taps == 9 || taps == 1, "taps must be 9 or 1");
   IGM_CHECK_ARG(cout >= 1 && cout <= kC, "cout must be in [1, 32]");
@@ -769,7 +776,7 @@ int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int
   IGM_CHECK_LAUNCH();
   const int entries = taps * kC * kC + kC;
   hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((entries + 63) / 64), dim3(256), 0, st, workspace,
-                     nblocks, taps, cout, dW_hwio, db);
+                     nblocks, taps, cout, accumulate, dW_hwio, db);
   IGM_CHECK_LAUNCH();
   return 0;
 }
@@ -777,7 +784,8 @@ int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int
 int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const float* f3,
                       const float* f4, int64_t fb, int64_t fr, int64_t fc, const int32_t* extents,
                       int32_t B, int32_t H, int32_t W, const void* dz1, int32_t nblocks,
-                      float* workspace, float* dW_hwio, float* db, void* stream) {
+                      float* workspace, int32_t accumulate, float* dW_hwio, float* db,
+                      void* stream) {
   IGM_CHECK_ARG(f0 && f1 && f2 && f3 && f4 && dz1 && workspace && dW_hwio && db, "null pointer");
   IGM_CHECK_ARG(nblocks >= 1 && B >= 1 && H >= 1 && W >= 1, "bad sizes");
   hipStream_t st = (hipStream_t)stream;
@@ -787,10 +795,10 @@ int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const f
   const int n = 9 * kCin * kC + kC;
   // the layer-1 partial layout is [blk][1440 + 32] = [blk][n]: a plain column sum
   hipLaunchKernelGGL(sum_partials_kernel, dim3((9 * kCin * kC + 63) / 64), dim3(256), 0, st,
-                     workspace, nblocks, n, 9 * kCin * kC, dW_hwio);
+                     workspace, nblocks, n, 9 * kCin * kC, accumulate, dW_hwio);
   IGM_CHECK_LAUNCH();
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, workspace + 9 * kCin * kC,
-                     nblocks, n, kC, db);
+                     nblocks, n, kC, accumulate, db);
   IGM_CHECK_LAUNCH();
   return 0;
 }

@@ -7,8 +7,9 @@
 // utils/gradient/compute_gradient.py:3-16, solve/solve.py:166-205.
 //
 // Work decomposition ("column marching"): a wavefront owns a strip of 64
-// node columns and a block of `rows` node rows.  Lane l holds node column
-// ic = 62*strip - 1 + l; lanes 1..62 own their column, lanes 0 and 63 are a
+// node columns and a block of consecutive node rows (the rows of a strip
+// are split evenly into as many blocks as give ~1 wave per SIMD in all).
+// Lane l holds node column ic = 62*strip - 1 + l; lanes 1..62 own their column, lanes 0 and 63 are a
 // one-node halo.  The wave walks up the rows: at step jc it holds node rows
 // jc and jc+1 of U and V (all Nz levels) in registers, computes the cell
 // (jc, ic) -- the cell whose lower-left node is (jc, ic) -- taking the right
@@ -39,7 +40,6 @@ constexpr int kPlanes = 7;     // q, w, gx, gy, cs, bx, by
 constexpr int kStripOut = 62;  // owned columns per wave
 constexpr int kPassBlock = 256;
 constexpr int kWavesPerBlock = kPassBlock / kWave;
-constexpr int kDefaultRows = 8;
 
 // --------------------------------------------------------- coefficients --
 //
@@ -129,22 +129,60 @@ constexpr int kRsrcWord3 = 0x00020000;      // raw buffer, gfx9 data format
 __device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
 }
-__device__ __forceinline__ float bload(rsrc_t r, uint32_t off, float) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+// voff: per-lane byte offset (kOOB masks the lane); soff: wave-uniform byte
+// offset carried in an SGPR (the level offset), so it costs no VALU add.
+// Either way the range check sees an offset >= kOOB for a masked lane.
+__device__ __forceinline__ float bload(rsrc_t r, uint32_t voff, uint32_t soff, float) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
-__device__ __forceinline__ double bload(rsrc_t r, uint32_t off, double) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+__device__ __forceinline__ double bload(rsrc_t r, uint32_t voff, uint32_t soff, double) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
   return __hiloint2double((int)v[1], (int)v[0]);
 }
-__device__ __forceinline__ void bstore(float x, rsrc_t r, uint32_t off) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, off, 0, 0);
+__device__ __forceinline__ void bstore(float x, rsrc_t r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, voff, soff, 0);
 }
-__device__ __forceinline__ void bstore(double x, rsrc_t r, uint32_t off) {
+__device__ __forceinline__ void bstore(double x, rsrc_t r, uint32_t voff, uint32_t soff) {
   typedef unsigned int u2 __attribute__((ext_vector_type(2)));
   u2 v;
   v[0] = (unsigned)__double2loint(x);
   v[1] = (unsigned)__double2hiint(x);
-  __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+}
+
+// The four Adam moments of one node and level, (m_U, v_U, m_V, v_V): one
+// 16-B access in float32, two in float64.
+__device__ __forceinline__ void mload(rsrc_t r, uint32_t voff, uint32_t soff, float (&m)[4]) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) m[i] = __uint_as_float(v[i]);
+}
+__device__ __forceinline__ void mload(rsrc_t r, uint32_t voff, uint32_t soff, double (&m)[4]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * h, soff, 0);
+    m[2 * h] = __hiloint2double((int)v[1], (int)v[0]);
+    m[2 * h + 1] = __hiloint2double((int)v[3], (int)v[2]);
+  }
+}
+__device__ __forceinline__ void mstore(const float (&m)[4], rsrc_t r, uint32_t voff, uint32_t soff) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  u4 v;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i] = __float_as_uint(m[i]);
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
+}
+__device__ __forceinline__ void mstore(const double (&m)[4], rsrc_t r, uint32_t voff, uint32_t soff) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    u4 v;
+    v[0] = (unsigned)__double2loint(m[2 * h]);
+    v[1] = (unsigned)__double2hiint(m[2 * h]);
+    v[2] = (unsigned)__double2loint(m[2 * h + 1]);
+    v[3] = (unsigned)__double2hiint(m[2 * h + 1]);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff + 16 * h, soff, 0);
+  }
 }
 
 template <typename T>
@@ -155,18 +193,16 @@ struct PassArgs {
   T* gV;
   T* Uo;
   T* Vo;
-  T* mU;
-  T* vU;
-  T* mV;
-  T* vV;
+  T* mom;         // (nbatch, Nz, Ny, Nx, 4) Adam moments, planar layout only
   const T* coef;
   const T* hscale;
   double* partials;
   const int32_t* stop;
   uint32_t vel_bytes;   // size of every velocity-shaped resource
+  uint32_t mom_bytes;
   uint32_t coef_bytes;
   uint32_t bs_bytes;    // batch stride of the velocity layout, bytes
-  int Ny, Nx, plane, rows, nstrips, nrowblk, blocks_per_batch;
+  int Ny, Nx, plane, nstrips, nrowblk, blocks_per_batch;
   T dz[IGM_MAX_NZ];   // layer thickness dzeta_l
   T idz[IGM_MAX_NZ];  // 1/dzeta_l
   T wz[IGM_MAX_NZ];   // level weight of the gravity term: (dzeta_{k-1} + dzeta_k)/2
@@ -192,12 +228,13 @@ energy_pass_kernel(const PassArgs<T> a) {
 
   if (!stopped && wid < a.nstrips * a.nrowblk) {
     const int strip = wid % a.nstrips;
-    const int rb = wid / a.nstrips;
+    const int rblk = wid / a.nstrips;
     const int ic = strip * kStripOut - 1 + lane;
     const bool col_ok = (ic >= 0) && (ic < a.Nx);
     const bool own = (lane >= 1) && (lane <= kStripOut) && col_ok;
-    const int j0 = rb * a.rows;
-    const int j1 = min(j0 + a.rows, a.Ny);
+    // node rows [j0, j1) of this wave: the Ny rows split evenly over nrowblk
+    const int j0 = (int)(((int64_t)rblk * a.Ny) / a.nrowblk);
+    const int j1 = (int)(((int64_t)(rblk + 1) * a.Ny) / a.nrowblk);
     const rsrc_t rU = make_rsrc(a.U, a.vel_bytes), rV = make_rsrc(a.V, a.vel_bytes);
     const rsrc_t rC = make_rsrc(a.coef, a.coef_bytes);
     const uint32_t bbase = (uint32_t)b * a.bs_bytes;
@@ -216,91 +253,100 @@ energy_pass_kernel(const PassArgs<T> a) {
       const uint32_t o = node_off(row, true);
 #pragma unroll
       for (int k = 0; k < NZ; ++k) {
-        u[k] = bload(rU, o + k * kstep, T());
-        v[k] = bload(rV, o + k * kstep, T());
+        u[k] = bload(rU, o, k * kstep, T());
+        v[k] = bload(rV, o, k * kstep, T());
       }
     };
 
-    T up[NZ], vp[NZ], uc[NZ], vc[NZ], cu[NZ], cv[NZ];
-    load_row(up, vp, j0 - 1);
-    load_row(uc, vc, j0);
+    // upper-corner adjoint contributions of the previous cell row
+    T cu[NZ], cv[NZ];
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
       cu[k] = T(0);
       cv[k] = T(0);
     }
 
-    for (int jc = j0 - 1; jc < j1; ++jc) {
-      T un[NZ], vn[NZ];
-      load_row(un, vn, jc + 2);  // prefetch for the next step
+    // coefficient planes of cell row jc (kOOB rows read 0)
+    const uint32_t ps = (uint32_t)a.plane * ES;
+    auto load_coef = [&](T (&c)[kPlanes], int jc) {
+      const uint32_t co = (col_ok && jc >= 0 && jc < a.Ny) ? cbase + (uint32_t)(jc * a.Nx + ic) * ES : kOOB;
+#pragma unroll
+      for (int p = 0; p < kPlanes; ++p) c[p] = bload(rC, co, p * ps, T());
+    };
 
-      const uint32_t co = (col_ok && jc >= 0) ? cbase + (uint32_t)(jc * a.Nx + ic) * ES : kOOB;
-      const uint32_t ps = (uint32_t)a.plane * ES;
-      const T q = bload(rC, co, T()), w = bload(rC, co + ps, T());
-      const T gx = bload(rC, co + 2 * ps, T()), gy = bload(rC, co + 3 * ps, T());
-      const T csl = bload(rC, co + 4 * ps, T());
-      const T bx = bload(rC, co + 5 * ps, T()), by = bload(rC, co + 6 * ps, T());
+    // float32 fused Adam: the moments of node row jc are loaded at the start
+    // of step jc and the update runs at its end, after the level sweep.
+    // (The float64 pass keeps its loads at the point of use, where its
+    // register budget stays within the VGPR file.)
+    constexpr bool kDeferAdam = (MODE == 2) && (sizeof(T) == 4);
+
+    // One step: node rows jc (up, vp) and jc+1 (uc, vc) and the coefficients
+    // of cell row jc (cc) are in registers, loaded one step earlier.  The
+    // step issues the loads of the next step first (vector memory returns in
+    // order, so the sweep below never waits on them), computes cell row jc
+    // and emits node row jc.
+    auto step = [&](int jc, const T (&up)[NZ], const T (&vp)[NZ], const T (&uc)[NZ],
+                    const T (&vc)[NZ], T (&un)[NZ], T (&vn)[NZ], const T (&cc)[kPlanes],
+                    T (&cn)[kPlanes]) {
       const bool emit_row = own && (jc >= j0);
       const uint32_t on = node_off(jc, emit_row);  // output node (jc, ic), masked
-      // Adam moments of the output node row, issued before the level sweep so
-      // their latency hides behind it (masked lanes read 0)
-      // (float32 only: the float64 pass keeps its loads at the point of use,
-      // where its register budget stays within the VGPR file without SGPR spills)
-      constexpr bool kPrefetchMoments = (MODE == 2) && (sizeof(T) == 4);
-      T mu_[NZ], vu_[NZ], mv_[NZ], vv_[NZ];
-      if constexpr (kPrefetchMoments) {
-        const rsrc_t rmu = make_rsrc(a.mU, a.vel_bytes), rvu = make_rsrc(a.vU, a.vel_bytes);
-        const rsrc_t rmv = make_rsrc(a.mV, a.vel_bytes), rvv = make_rsrc(a.vV, a.vel_bytes);
+      // moments of node (jc, ic): 4 x the velocity node offset, plus the batch
+      const uint32_t om = emit_row ? (uint32_t)b * (uint32_t)(NZ * a.plane) * 4u * ES +
+                                         (uint32_t)(jc * a.Nx + ic) * 4u * ES
+                                   : kOOB;
+      const uint32_t mstep = 4u * kstep;
+      const rsrc_t rM = make_rsrc(a.mom, a.mom_bytes);
+      T mo[NZ][4], gu_[NZ], gv_[NZ];
+      if constexpr (kDeferAdam) {
 #pragma unroll
-        for (int k = 0; k < NZ; ++k) {
-          mu_[k] = bload(rmu, on + k * kstep, T());
-          vu_[k] = bload(rvu, on + k * kstep, T());
-          mv_[k] = bload(rmv, on + k * kstep, T());
-          vv_[k] = bload(rvv, on + k * kstep, T());
-        }
+        for (int k = 0; k < NZ; ++k) mload(rM, om, k * mstep, mo[k]);
       }
+      load_coef(cn, jc + 1);     // next step's coefficients
+      load_row(un, vn, jc + 2);  // next step's upper node row
+      // keep the loads above ahead of the sweep (the scheduler would sink them)
+      __builtin_amdgcn_sched_barrier(0);
+      const T q = cc[0], w = cc[1], gx = cc[2], gy = cc[3], csl = cc[4], bx = cc[5], by = cc[6];
 
-      // scatter of the adjoint of level k to the 4 corner nodes, then the
-      // output of node (jc, ic), level k (masked stores)
+      // legacy-Adam update of node (jc, ic), level k, from its gradient
+      auto adam = [&](int k, T g_u, T g_v, T (&m)[4]) {
+        m[0] += (g_u - m[0]) * a.omb1;
+        m[1] += (g_u * g_u - m[1]) * a.omb2;
+        m[2] += (g_v - m[2]) * a.omb1;
+        m[3] += (g_v * g_v - m[3]) * a.omb2;
+        bstore(up[k] - (m[0] * a.alpha) * rcp(sqrt_(m[1]) + a.eps), make_rsrc(a.Uo, a.vel_bytes), on,
+               k * kstep);
+        bstore(vp[k] - (m[2] * a.alpha) * rcp(sqrt_(m[3]) + a.eps), make_rsrc(a.Vo, a.vel_bytes), on,
+               k * kstep);
+        mstore(m, rM, om, k * mstep);
+      };
+
+      // Adjoint scatter of level k to the 4 corner nodes, then the output of
+      // node (jc, ic), level k (masked stores).  With A = gs - gx and
+      // B = gs + gx, the corners get A -+ gy (left) and B -+ gy (right); the
+      // right ones arrive from lane l-1: node row jc receives
+      // X - Z, node row jc+1 (carried) X + Z, X = A + B(l-1), Z = gy + gy(l-1).
+      auto scatter = [&](T& carry, T gx_, T gy_, T gs_) -> T {
+        const T A = gs_ - gx_, B = gs_ + gx_;
+        const T X = A + lane_from_left(B);
+        const T Z = gy_ + lane_from_left(gy_);
+        const T g = (carry - Z) + X;
+        carry = X + Z;
+        return g;
+      };
       auto emit = [&](int k, T gux, T guy, T gsu, T gvx, T gvy, T gsv) {
-        const uint32_t ok = on + k * kstep;
-        {
-          const T A = gsu - gux, B = gsu + gux;
-          const T c00 = A - guy, c10 = A + guy, c01 = B - guy, c11 = B + guy;
-          const T g = cu[k] + c00 + lane_from_left(c01);
-          cu[k] = c10 + lane_from_left(c11);
-          if constexpr (MODE == 1) {
-            bstore(g, make_rsrc(a.gU, a.vel_bytes), ok);
-          } else if constexpr (MODE == 2) {
-            const rsrc_t rm = make_rsrc(a.mU, a.vel_bytes), rv = make_rsrc(a.vU, a.vel_bytes);
-            T m, v;
-            if constexpr (kPrefetchMoments) { m = mu_[k]; v = vu_[k]; }
-            else { m = bload(rm, ok, T()); v = bload(rv, ok, T()); }
-            m += (g - m) * a.omb1;
-            v += (g * g - v) * a.omb2;
-            bstore(up[k] - (m * a.alpha) * rcp(sqrt_(v) + a.eps), make_rsrc(a.Uo, a.vel_bytes), ok);
-            bstore(m, rm, ok);
-            bstore(v, rv, ok);
-          }
-        }
-        {
-          const T A = gsv - gvx, B = gsv + gvx;
-          const T c00 = A - gvy, c10 = A + gvy, c01 = B - gvy, c11 = B + gvy;
-          const T g = cv[k] + c00 + lane_from_left(c01);
-          cv[k] = c10 + lane_from_left(c11);
-          if constexpr (MODE == 1) {
-            bstore(g, make_rsrc(a.gV, a.vel_bytes), ok);
-          } else if constexpr (MODE == 2) {
-            const rsrc_t rm = make_rsrc(a.mV, a.vel_bytes), rv = make_rsrc(a.vV, a.vel_bytes);
-            T m, v;
-            if constexpr (kPrefetchMoments) { m = mv_[k]; v = vv_[k]; }
-            else { m = bload(rm, ok, T()); v = bload(rv, ok, T()); }
-            m += (g - m) * a.omb1;
-            v += (g * g - v) * a.omb2;
-            bstore(vp[k] - (m * a.alpha) * rcp(sqrt_(v) + a.eps), make_rsrc(a.Vo, a.vel_bytes), ok);
-            bstore(m, rm, ok);
-            bstore(v, rv, ok);
-          }
+        const uint32_t sk = k * kstep;
+        const T g_u = scatter(cu[k], gux, guy, gsu);
+        const T g_v = scatter(cv[k], gvx, gvy, gsv);
+        if constexpr (MODE == 1) {
+          bstore(g_u, make_rsrc(a.gU, a.vel_bytes), on, sk);
+          bstore(g_v, make_rsrc(a.gV, a.vel_bytes), on, sk);
+        } else if constexpr (kDeferAdam) {
+          gu_[k] = g_u;
+          gv_[k] = g_v;
+        } else if constexpr (MODE == 2) {
+          T m[4];
+          mload(rM, om, k * mstep, m);
+          adam(k, g_u, g_v, m);
         }
       };
 
@@ -310,7 +356,7 @@ energy_pass_kernel(const PassArgs<T> a) {
       // hh2 = 1/(4 dx)^2 restores the strain rate where it is needed.
       T p_ux = 0, p_uy = 0, p_vx = 0, p_vy = 0, p_su = 0, p_sv = 0;  // level k-1
       T g_ux = 0, g_uy = 0, g_vx = 0, g_vy = 0, g_su = 0, g_sv = 0;  // its adjoint
-      T c_sh = 0, c_gr = 0, c_sl = 0;
+      T c_sh = 0, c_sl = 0, wsu = 0, wsv = 0;
 #pragma unroll
       for (int k = 0; k < NZ; ++k) {
         const T su = up[k] + uc[k], du = uc[k] - up[k];
@@ -322,7 +368,8 @@ energy_pass_kernel(const PassArgs<T> a) {
         const T sU = su + su_r, sV = sv + sv_r;  // 4 x node average
         T n_ux = 0, n_uy = 0, n_vx = 0, n_vy = 0;
         T n_su = a.wz[k] * gx, n_sv = a.wz[k] * gy;  // gravity (linear)
-        c_gr += a.wz[k] * (gx * sU + gy * sV);
+        wsu += a.wz[k] * sU;
+        wsv += a.wz[k] * sV;
         if (k == 0) {  // Weertman sliding on the basal level
           const T ub = T(0.25) * sU, vb = T(0.25) * sV;
           const T c2 = ub * bx + vb * by;
@@ -377,16 +424,34 @@ energy_pass_kernel(const PassArgs<T> a) {
       }
       emit(NZ - 1, g_ux, g_uy, g_su, g_vx, g_vy, g_sv);
 
+      if constexpr (kDeferAdam) {
+        // keep the updates after the sweep (the scheduler would hoist them
+        // into it, right after each gradient, and wait there for the moments)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) adam(k, gu_[k], gv_[k], mo[k]);
+      }
+
       if (emit_row) {
         e_sh += (double)(w * c_sh);
         e_sl += (double)c_sl;
-        e_gr += (double)c_gr;
+        e_gr += (double)(gx * wsu + gy * wsv);
       }
-#pragma unroll
-      for (int k = 0; k < NZ; ++k) {
-        up[k] = uc[k]; vp[k] = vc[k];
-        uc[k] = un[k]; vc[k] = vn[k];
-      }
+    };
+
+    // three row and coefficient buffers in rotation, unrolled so that no
+    // register moves
+    T bu0[NZ], bv0[NZ], bu1[NZ], bv1[NZ], bu2[NZ], bv2[NZ];
+    T cf0[kPlanes], cf1[kPlanes], cf2[kPlanes];
+    load_row(bu0, bv0, j0 - 1);
+    load_row(bu1, bv1, j0);
+    load_coef(cf0, j0 - 1);
+    for (int jc = j0 - 1; jc < j1; jc += 3) {
+      step(jc, bu0, bv0, bu1, bv1, bu2, bv2, cf0, cf1);
+      if (jc + 1 >= j1) break;
+      step(jc + 1, bu1, bv1, bu2, bv2, bu0, bv0, cf1, cf2);
+      if (jc + 2 >= j1) break;
+      step(jc + 2, bu2, bv2, bu0, bv0, bu1, bv1, cf2, cf0);
     }
   }
 
@@ -416,23 +481,33 @@ energy_pass_kernel(const PassArgs<T> a) {
   }
 }
 
-int rows_per_wave() {
-  static int rows = [] {
-    const char* e = getenv("IGM_ENERGY_ROWS");
-    const int r = e ? atoi(e) : kDefaultRows;
-    return (r >= 1 && r <= 1024) ? r : kDefaultRows;
+// Waves per launch the row split aims at: 4 per CU of MI355X's 256, one per
+// SIMD, all resident at once, so that every SIMD gets one equally long wave
+// (measured at 1000 x 1000: 1024 waves beat 2048-4096, whose extra halo rows
+// cost more than the latency a second wave would hide).  IGM_ENERGY_WAVES
+// overrides it (for tuning sweeps).
+constexpr int kTargetWaves = 1024;
+constexpr int kMinRows = 4;  // fewer rows per wave would be mostly halo
+
+int target_waves() {
+  static int n = [] {
+    const char* e = getenv("IGM_ENERGY_WAVES");
+    const int r = e ? atoi(e) : kTargetWaves;
+    return (r >= 64 && r <= (1 << 20)) ? r : kTargetWaves;
   }();
-  return rows;
+  return n;
 }
 
 struct PassGeom {
-  int rows, nstrips, nrowblk, blocks;
+  int nstrips, nrowblk, blocks;
 };
-inline PassGeom pass_geom(int Ny, int Nx) {
+inline PassGeom pass_geom(int nbatch, int Ny, int Nx) {
   PassGeom g;
-  g.rows = rows_per_wave();
   g.nstrips = (Nx + kStripOut - 1) / kStripOut;
-  g.nrowblk = (Ny + g.rows - 1) / g.rows;
+  const int64_t per = (int64_t)g.nstrips * nbatch;
+  int64_t want = (target_waves() + per / 2) / per;
+  const int64_t most = Ny / kMinRows > 1 ? Ny / kMinRows : 1;
+  g.nrowblk = (int)(want < 1 ? 1 : (want > most ? most : want));
   const int waves = g.nstrips * g.nrowblk;
   g.blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
   return g;
@@ -551,9 +626,8 @@ void launch_pass_nz(int nz, dim3 grid, hipStream_t st, const PassArgs<T>& a, boo
 template <typename T>
 int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T* V, int64_t bs,
               int64_t ks, int64_t rs, int64_t cs, const T* coef, const T* hscale,
-              const double* dzeta, const IgmPhysics* phys, T* gU, T* gV, T* Uo, T* Vo, T* mU,
-              T* vU, T* mV, T* vV, const IgmAdam* adam, double* partials, const int32_t* stop,
-              void* stream) {
+              const double* dzeta, const IgmPhysics* phys, T* gU, T* gV, T* Uo, T* Vo, T* mom,
+              const IgmAdam* adam, double* partials, const int32_t* stop, void* stream) {
   IGM_CHECK_ARG(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
   IGM_CHECK_ARG(U && V && coef && hscale && dzeta && phys && partials, "null pointer");
   IGM_CHECK_ARG(nbatch >= 1 && Ny >= 2 && Nx >= 2, "need nbatch >= 1 and a grid of at least 2 x 2");
@@ -561,7 +635,7 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
   IGM_CHECK_ARG(nbatch <= 65535, "nbatch must be <= 65535");
   if (mode == 1) IGM_CHECK_ARG(gU && gV, "mode 1 needs gU and gV");
   if (mode == 2) {
-    IGM_CHECK_ARG(Uo && Vo && mU && vU && mV && vV && adam, "mode 2 needs Uout, Vout, m, v and adam");
+    IGM_CHECK_ARG(Uo && Vo && mom && adam, "mode 2 needs Uout, Vout, moments and adam");
     IGM_CHECK_ARG(Uo != U && Vo != V, "Uout/Vout must not alias U/V");
   }
   // supported layouts: planar (Nz, Ny, Nx) or node-interleaved (Ny, Nx, 2 Nz)
@@ -570,24 +644,27 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
   if (cs == 1 && rs == Nx && ks == plane) inter = 0;
   if (ks == 1 && cs == 2 * nz && rs == (int64_t)Nx * 2 * nz && V == U + nz) inter = 1;
   IGM_CHECK_ARG(inter >= 0, "unsupported velocity layout (need planar (Nz,Ny,Nx) or interleaved (Ny,Nx,2Nz))");
+  IGM_CHECK_ARG(mode != 2 || inter == 0, "mode 2 needs the planar velocity layout");
   const int64_t per = inter ? plane * 2 * nz : plane * nz;  // elements of one batch entry
   if (nbatch > 1) IGM_CHECK_ARG(bs >= per, "batch stride smaller than one entry");
   const int64_t span = (int64_t)(nbatch - 1) * (nbatch > 1 ? bs : 0) + per;
   const int64_t vel_bytes = span * (int64_t)sizeof(T);
   const int64_t coef_bytes = (int64_t)nbatch * kPlanes * plane * (int64_t)sizeof(T);
-  IGM_CHECK_ARG(vel_bytes < (int64_t)kOOB && coef_bytes < (int64_t)kOOB,
+  const int64_t mom_bytes = (int64_t)nbatch * nz * plane * 4 * (int64_t)sizeof(T);
+  IGM_CHECK_ARG(vel_bytes < (int64_t)kOOB && coef_bytes < (int64_t)kOOB && mom_bytes < (int64_t)kOOB,
                 "problem too large for 32-bit buffer offsets (2 GiB per buffer)");
   PassArgs<T> a{};
   a.U = U; a.V = V; a.gU = gU; a.gV = gV; a.Uo = Uo; a.Vo = Vo;
-  a.mU = mU; a.vU = vU; a.mV = mV; a.vV = vV;
+  a.mom = mom;
   a.coef = coef; a.hscale = hscale; a.partials = partials; a.stop = stop;
   a.vel_bytes = (uint32_t)vel_bytes;
   a.coef_bytes = (uint32_t)coef_bytes;
+  a.mom_bytes = (uint32_t)mom_bytes;
   a.bs_bytes = (uint32_t)((nbatch > 1 ? bs : 0) * (int64_t)sizeof(T));
   a.plane = (int)plane;
   a.Ny = Ny; a.Nx = Nx;
-  const PassGeom g = pass_geom(Ny, Nx);
-  a.rows = g.rows; a.nstrips = g.nstrips; a.nrowblk = g.nrowblk; a.blocks_per_batch = g.blocks;
+  const PassGeom g = pass_geom(nbatch, Ny, Nx);
+  a.nstrips = g.nstrips; a.nrowblk = g.nrowblk; a.blocks_per_batch = g.blocks;
   for (int k = 0; k < IGM_MAX_NZ; ++k) { a.dz[k] = a.idz[k] = a.wz[k] = T(0); }
   for (int l = 0; l < nz - 1; ++l) {
     IGM_CHECK_ARG(dzeta[l] > 0.0, "dzeta must be positive");
@@ -622,7 +699,7 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
 #define IGM_MODE_LAYOUT(M, I) \
   if (mode == M && inter == I) launch_pass_nz<T, M, I>(nz, grid, st, a, &ok);
   IGM_MODE_LAYOUT(0, 0) IGM_MODE_LAYOUT(1, 0) IGM_MODE_LAYOUT(2, 0)
-  IGM_MODE_LAYOUT(0, 1) IGM_MODE_LAYOUT(1, 1) IGM_MODE_LAYOUT(2, 1)
+  IGM_MODE_LAYOUT(0, 1) IGM_MODE_LAYOUT(1, 1)
 #undef IGM_MODE_LAYOUT
   IGM_CHECK_ARG(ok, "nz not compiled (supported: 2,3,4,5,6,8,10,12,16,20)");
   IGM_CHECK_LAUNCH();
@@ -703,7 +780,7 @@ const char* igm_last_error(void) { return igm::last_error(); }
 
 int64_t igm_energy_workspace_doubles(int32_t nbatch, int32_t Ny, int32_t Nx) {
   if (nbatch < 1 || Ny < 2 || Nx < 2) return 0;
-  const igm::PassGeom g = igm::pass_geom(Ny, Nx);
+  const igm::PassGeom g = igm::pass_geom(nbatch, Ny, Nx);
   return 3 * (int64_t)nbatch * g.blocks;
 }
 
@@ -728,23 +805,22 @@ int igm_energy_pass_f32(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, in
                         const float* U, const float* V, int64_t bstride, int64_t kstride,
                         int64_t rstride, int64_t cstride, const float* coef, const float* hscale,
                         const double* dzeta, const IgmPhysics* phys, float* gU, float* gV,
-                        float* Uout, float* Vout, float* mU, float* vU, float* mV, float* vV,
-                        const IgmAdam* adam, double* partials, const int32_t* stop_flag,
-                        void* stream) {
+                        float* Uout, float* Vout, float* moments, const IgmAdam* adam,
+                        double* partials, const int32_t* stop_flag, void* stream) {
   return igm::pass_impl<float>(mode, nz, nbatch, Ny, Nx, U, V, bstride, kstride, rstride, cstride,
-                               coef, hscale, dzeta, phys, gU, gV, Uout, Vout, mU, vU, mV, vV, adam,
+                               coef, hscale, dzeta, phys, gU, gV, Uout, Vout, moments, adam,
                                partials, stop_flag, stream);
 }
 int igm_energy_pass_f64(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
                         const double* U, const double* V, int64_t bstride, int64_t kstride,
                         int64_t rstride, int64_t cstride, const double* coef,
                         const double* hscale, const double* dzeta, const IgmPhysics* phys,
-                        double* gU, double* gV, double* Uout, double* Vout, double* mU,
-                        double* vU, double* mV, double* vV, const IgmAdam* adam,
-                        double* partials, const int32_t* stop_flag, void* stream) {
+                        double* gU, double* gV, double* Uout, double* Vout, double* moments,
+                        const IgmAdam* adam, double* partials, const int32_t* stop_flag,
+                        void* stream) {
   return igm::pass_impl<double>(mode, nz, nbatch, Ny, Nx, U, V, bstride, kstride, rstride,
-                                cstride, coef, hscale, dzeta, phys, gU, gV, Uout, Vout, mU, vU, mV,
-                                vV, adam, partials, stop_flag, stream);
+                                cstride, coef, hscale, dzeta, phys, gU, gV, Uout, Vout, moments,
+                                adam, partials, stop_flag, stream);
 }
 
 int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int32_t Nx,
@@ -753,7 +829,7 @@ int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int3
   IGM_CHECK_ARG(partials && out, "null pointer");
   IGM_CHECK_ARG(nbatch >= 1 && Ny >= 2 && Nx >= 2, "bad sizes");
   IGM_CHECK_ARG(es == nullptr || nbatch == 1, "early stopping needs nbatch == 1");
-  const igm::PassGeom g = igm::pass_geom(Ny, Nx);
+  const igm::PassGeom g = igm::pass_geom(nbatch, Ny, Nx);
   hipLaunchKernelGGL(igm::energy_finalize_kernel, dim3(nbatch), dim3(igm::kFinBlock), 0,
                      (hipStream_t)stream, partials, g.blocks, out, es, cost_hist, iter);
   IGM_CHECK_LAUNCH();

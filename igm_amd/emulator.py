@@ -134,8 +134,10 @@ class CNN:
         return (Y, acts) if keep else Y
 
     # ----------------------------------------------------------- backward --
-    def backward(self, dY, acts, fields, extents=None, nblocks=512):
-        """Gradient of sum(dY * Y) w.r.t. all parameters, into self.g."""
+    def backward(self, dY, acts, fields, extents=None, nblocks=512, accumulate=False):
+        """Gradient of sum(dY * Y) w.r.t. all parameters, into self.g (added
+        to it when ``accumulate``)."""
+        acc = 1 if accumulate else 0
         fptrs, (fb, fr, fc), B, H, W = fields
         ext = L.ptr(extents)
         st = L.stream_ptr()
@@ -148,16 +150,16 @@ class CNN:
         L.call("igm_conv1x1_head_bwd", L.ptr(dY), L.ptr(acts[nl - 1]), L.ptr(self.kernel(nl)), ext, B, H,
                W, self.cout, L.ptr(dz), L.ptr(dYb), st)
         L.call("igm_conv_wgrad_bf16", 1, L.ptr(acts[nl - 1]), L.ptr(dYb), ext, B, H, W, self.cout,
-               nblocks, L.ptr(ws), L.ptr(self.kernel(nl, self.g)), L.ptr(self.bias(nl, self.g)), st)
+               nblocks, L.ptr(ws), acc, L.ptr(self.kernel(nl, self.g)), L.ptr(self.bias(nl, self.g)), st)
         below = dYb  # reuse as the ping-pong buffer
         for li in range(nl - 1, 0, -1):  # layer index li (0-based) is a 3x3 MFMA layer
             a_in = acts[li - 1]
             L.call("igm_conv_wgrad_bf16", 9, L.ptr(a_in), L.ptr(dz), ext, B, H, W, NF, nblocks,
-                   L.ptr(ws), L.ptr(self.kernel(li, self.g)), L.ptr(self.bias(li, self.g)), st)
+                   L.ptr(ws), acc, L.ptr(self.kernel(li, self.g)), L.ptr(self.bias(li, self.g)), st)
             L.call("igm_conv3x3_dgrad_bf16", L.ptr(dz), L.ptr(self.dgr[li - 1]), L.ptr(a_in), ext, B, H,
                    W, L.ptr(below), st)
             dz, below = below, dz
-        L.call("igm_conv_in_wgrad", *fptrs, fb, fr, fc, ext, B, H, W, L.ptr(dz), nblocks, L.ptr(ws),
+        L.call("igm_conv_in_wgrad", *fptrs, fb, fr, fc, ext, B, H, W, L.ptr(dz), nblocks, L.ptr(ws), acc,
                L.ptr(self.kernel(0, self.g)), L.ptr(self.bias(0, self.g)), st)
         return self.g
 

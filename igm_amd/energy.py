@@ -99,8 +99,10 @@ class EnergyOperator:
                C.addressof(self.phys), L.ptr(self.coef), L.ptr(self.hscale), L.stream_ptr())
 
     # -- raw passes ---------------------------------------------------------
-    def run_pass(self, mode, U, V=None, gU=None, gV=None, Uo=None, Vo=None, mU=None, vU=None,
-                 mV=None, vV=None, adam=None, stop_flag=None):
+    def run_pass(self, mode, U, V=None, gU=None, gV=None, Uo=None, Vo=None, moments=None, adam=None,
+                 stop_flag=None):
+        """One pass (mode 0 energy, 1 + gradient, 2 + Adam step).  ``moments``
+        (mode 2): (B, Nz, Ny, Nx, 4) Adam slots (m_U, v_U, m_V, v_V)."""
         up, vp, bs, ks, rs, cs = velocity_strides(U, V, self.nz)
         if gU is not None and gV is None:  # gradient in Y layout
             gu, gv = velocity_strides(gU, None, self.nz)[:2]
@@ -108,8 +110,8 @@ class EnergyOperator:
             gu, gv = L.ptr(gU), L.ptr(gV)
         L.call(f"igm_energy_pass_{self.sfx}", mode, self.nz, self.B, self.Ny, self.Nx, up, vp,
                bs, ks, rs, cs, L.ptr(self.coef), L.ptr(self.hscale), C.addressof(self.dzeta),
-               C.addressof(self.phys), gu, gv, L.ptr(Uo), L.ptr(Vo), L.ptr(mU), L.ptr(vU),
-               L.ptr(mV), L.ptr(vV), C.addressof(adam) if adam is not None else None,
+               C.addressof(self.phys), gu, gv, L.ptr(Uo), L.ptr(Vo), L.ptr(moments),
+               C.addressof(adam) if adam is not None else None,
                L.ptr(self.partials), stop_flag, L.stream_ptr())
 
     def finalize(self, out=None, es=None, cost_hist=None, it=0):

@@ -13,8 +13,10 @@ libigm_amd.so; this module is host control flow.
 Methods (cfg.processes.iceflow.method):
 * ``solved``     -- the plain energy solver (solve/solve.py:169-188);
 * ``emulated``   -- the CNN emulator with periodic physics-informed
-  retraining (emulate/emulate.py:116-281), plus an energy-gap fallback to
-  the solver (absent from the reference; cfg.processes.iceflow.emulator.fallback);
+  retraining (emulate/emulate.py:116-281), plus an optional warm-started
+  solver polish of the emulated field (cfg.processes.iceflow.emulator.refine)
+  and an energy-gap fallback to the solver
+  (cfg.processes.iceflow.emulator.fallback), both absent from the reference;
 * ``diagnostic`` -- emulated, plus a periodic warm-started solve that logs
   the emulator-vs-solver misfit (diagnostic/diagnostic.py:16-84).
 """
@@ -165,6 +167,8 @@ def update(cfg, state):
         if (ice.emulator.retrain_freq > 0) & (state.it > 0):
             emu.update_iceflow_emulator(cfg, state, state.it, pertubate=ice.emulator.pertubate)
         emu.update_iceflow_emulated(cfg, state)
+        if ice.method == "emulated" and int(cfgmod.get(ice, "emulator.refine.its", 0)) > 0:
+            refine_emulated(cfg, state)
         if ice.method == "emulated" and cfgmod.get(ice, "emulator.fallback.enabled", False):
             fallback_check(cfg, state)
         if ice.method == "diagnostic":
@@ -177,6 +181,29 @@ def finalize(cfg, state):
     """iceflow.py:116-119."""
     if cfg.processes.iceflow.emulator.save_model:
         emu.save_iceflow_model(cfg, state)
+
+
+# ------------------------------------------------------------- refine --
+
+def refine_emulated(cfg, state):
+    """Polish the emulated field: ``refine.its`` plain-solver iterations
+    warm-started from state.U, state.V with a fresh legacy-Adam state, step
+    ``refine.lr`` for the first ``flat_frac`` of them, then a geometric decay
+    to ``lr_end``; the outputs are rebuilt from the polished field.  A
+    network whose error is a few percent reaches the solver's accuracy in a
+    small fraction of the iterations a solve from zero needs."""
+    ice = cfg.processes.iceflow
+    rf = ice.emulator.refine
+    n = int(rf.its)
+    if n <= 0:
+        return
+    op = energy_operator(cfg, state)
+    n_flat = int(float(rf.flat_frac) * n)
+    res = slv.solve(op, state.U, state.V, n, slv.LegacyAdamState(lr=float(rf.lr)), early_stopping=False,
+                    lr_schedule=slv.adam_schedule(float(rf.lr), n_flat, n - n_flat, float(rf.lr_end)))
+    out = velocity_outputs(res.U, res.V, state.thk, state.vert_weight, int(ice.numerics.Nz))
+    _set_outputs(cfg, state, out)
+    state.COST_REFINED = float(res.costs[-1])
 
 
 # ----------------------------------------------------------- fallback --

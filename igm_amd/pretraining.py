@@ -65,61 +65,152 @@ def augment(t, code):
 
 
 class Batcher:
-    def __init__(self, pool, batch, seed, dev):
-        self.pool, self.batch, self.dev = pool, batch, dev
-        self.rng = np.random.default_rng(seed)
-        self.smax = max(max(p.shape[0], p.shape[1]) for p in pool)
+    """Batches of ``batch`` pool patches (random picks, random flips and
+    transposition).  ``buckets`` > 1 sorts a batch by patch area and splits
+    it into that many sub-batches, each zero-padded only to its own largest
+    patch: the gradients of the sub-batches add up to the one of the whole
+    batch (padding is masked), with less padded work."""
 
-    def next(self):
-        idx = self.rng.integers(0, len(self.pool), self.batch)
-        codes = self.rng.integers(0, 8, self.batch)
-        parts = [augment(self.pool[i], int(c)) for i, c in zip(idx, codes)]
+    def __init__(self, pool, batch, seed, dev, buckets=1):
+        self.pool, self.batch, self.dev = pool, batch, dev
+        self.buckets = max(1, min(int(buckets), batch))
+        self.rng = np.random.default_rng(seed)
+
+    def _pack(self, parts):
         H = max(p.shape[0] for p in parts)
         W = max(p.shape[1] for p in parts)
-        X = torch.zeros((self.batch, H, W, 5), dtype=torch.float32, device=self.dev)
-        ext = torch.empty((self.batch, 2), dtype=torch.int32, pin_memory=True)
+        X = torch.zeros((len(parts), H, W, 5), dtype=torch.float32, device=self.dev)
+        ext = torch.empty((len(parts), 2), dtype=torch.int32, pin_memory=True)
         for b, p in enumerate(parts):
             X[b, :p.shape[0], :p.shape[1]] = p
             ext[b, 0], ext[b, 1] = p.shape[0], p.shape[1]
         # pinned host buffer + non-blocking copy: no host/device synchronisation per step
         return X, ext.to(self.dev, non_blocking=True)
 
+    def next(self):
+        """List of (X (b, H, W, 5), extents (b, 2) int32) sub-batches."""
+        idx = self.rng.integers(0, len(self.pool), self.batch)
+        codes = self.rng.integers(0, 8, self.batch)
+        parts = [augment(self.pool[i], int(c)) for i, c in zip(idx, codes)]
+        if self.buckets == 1:
+            return [self._pack(parts)]
+        parts.sort(key=lambda p: p.shape[0] * p.shape[1])
+        cuts = np.linspace(0, len(parts), self.buckets + 1).round().astype(int)
+        return [self._pack(parts[c0:c1]) for c0, c1 in zip(cuts[:-1], cuts[1:]) if c1 > c0]
 
-def train(cnn, pool, steps, dzeta, phys, batch=32, lr=1e-4, lr_end=None, clipnorm=1.0, epsilon=1e-7,
-          seed=3, log_every=100, val=None, log=print):
-    """Energy-as-loss training; returns the list of (step, mean COST per entry).
 
-    lr decays geometrically from ``lr`` to ``lr_end`` (constant if None).
-    ``val(cnn)`` (optional) is called every log_every steps and its result logged."""
-    dev = cnn.device
-    bt = Batcher(pool, batch, seed, dev)
+class CropBatcher:
+    """Batches of ``batch`` square crops of side ``crop`` from whole fields
+    (random field, offset, flips and transposition): equal-size entries, so
+    no padding and no extents."""
+
+    def __init__(self, fields, batch, crop, seed, dev):
+        self.fields, self.batch, self.crop, self.dev = fields, batch, int(crop), dev
+        for f in fields:
+            assert f.shape[0] >= self.crop and f.shape[1] >= self.crop, "crop larger than a field"
+        self.rng = np.random.default_rng(seed)
+
+    def next(self):
+        c = self.crop
+        X = torch.empty((self.batch, c, c, 5), dtype=torch.float32, device=self.dev)
+        for b in range(self.batch):
+            f = self.fields[int(self.rng.integers(len(self.fields)))]
+            j0 = int(self.rng.integers(0, f.shape[0] - c + 1))
+            i0 = int(self.rng.integers(0, f.shape[1] - c + 1))
+            X[b] = augment(f[j0:j0 + c, i0:i0 + c], int(self.rng.integers(0, 8)))
+        return [(X, None)]
+
+
+class Validator:
+    """Relative velocity error of the emulator on one whole field against
+    the field's converged float64 energy minimiser (solver.converged_solution)."""
+
+    def __init__(self, field, dzeta, phys, dev, n_flat=8000, n_decay=20000):
+        from .solver import converged_solution
+        f32 = {k: torch.as_tensor(field[k], device=dev) for k in KEYS}
+        f64 = {k: v.double() for k, v in f32.items()}
+        op32 = EnergyOperator(dzeta, phys, fields=f32)
+        op64 = EnergyOperator(dzeta, phys, fields=f64, dtype=torch.float64)
+        self.nz = len(dzeta) + 1
+        z = torch.zeros((self.nz,) + tuple(f32["thk"].shape), dtype=torch.float32, device=dev)
+        self.Ur, self.Vr, self.J, self.change = converged_solution(op32, op64, z, z.clone(), n_flat, n_decay)
+        self.mask = f32["thk"] > 0
+        self._keep = [f32[k] for k in KEYS]
+        self.fields = EM.planar_fields(self._keep)
+
+    def __call__(self, cnn):
+        from .solver import rel_err
+        Y = cnn.forward(self.fields)
+        U = Y[0, ..., :self.nz].permute(2, 0, 1)
+        V = Y[0, ..., self.nz:].permute(2, 0, 1)
+        return rel_err(U, V, self.Ur, self.Vr, self.mask)
+
+
+def train(cnn, batcher, dzeta, phys, steps=None, seconds=None, lr=1e-4, lr_end=None, clipnorm=1.0,
+          epsilon=1e-7, log_every=100, val=None, on_best=None, log=print):
+    """Energy-as-loss training for ``steps`` steps or ``seconds`` of wall
+    time (whichever is given); returns the list of (step, mean COST per
+    entry[, validation error]).
+
+    lr decays geometrically from ``lr`` to ``lr_end`` over the run (constant
+    if None).  ``val(cnn)`` (optional) is evaluated every log_every steps;
+    ``on_best(cnn, v)`` is called when it improves."""
+    assert (steps is None) != (seconds is None), "give steps or seconds"
     hist = []
+    best = float("inf")
     t0 = time.time()
-    for s in range(steps):
-        X, ext = bt.next()
-        op = EnergyOperator(dzeta, phys, X=X, extents=ext)
-        fields = EM.nhwc_fields(X)
-        Y, acts = cnn.forward(fields, extents=ext, keep=True)
-        e, gY = op.energy_grad(Y)
-        cnn.backward(gY, acts, fields, extents=ext)
-        lr_s = lr if lr_end is None else lr * (lr_end / lr) ** (s / max(1, steps - 1))
+    s = 0
+    while True:
+        frac = (s / max(1, steps - 1)) if steps is not None else (time.time() - t0) / seconds
+        if (steps is not None and s >= steps) or (seconds is not None and frac >= 1.0):
+            break
+        costs = []
+        for k, (X, ext) in enumerate(batcher.next()):
+            op = EnergyOperator(dzeta, phys, X=X, extents=ext)
+            fields = EM.nhwc_fields(X)
+            Y, acts = cnn.forward(fields, extents=ext, keep=True)
+            e, gY = op.energy_grad(Y)
+            cnn.backward(gY, acts, fields, extents=ext, accumulate=k > 0)
+            costs.append(e[:, 3])
+        lr_s = lr if lr_end is None else lr * (lr_end / lr) ** min(1.0, frac)
         cnn.adam_step(lr_s, clipnorm=clipnorm, epsilon=epsilon)
-        if (s + 1) % log_every == 0 or s == 0:
-            c = float(e[:, 3].mean())
-            hist.append((s + 1, c))
-            msg = f"step {s + 1}: mean COST {c:.5f} lr {lr_s:.2e} ({time.time() - t0:.1f}s)"
+        s += 1
+        if s % log_every == 0 or s == 1:
+            c = float(torch.cat(costs).mean())
+            rec = [s, c]
+            msg = f"step {s}: mean COST {c:.5f} lr {lr_s:.2e} ({time.time() - t0:.1f}s)"
             if val is not None:
-                msg += f" | val {val(cnn)}"
+                v = float(val(cnn))
+                rec.append(v)
+                msg += f" | val {v:.4f}"
+                if v < best:
+                    best = v
+                    if on_best is not None:
+                        on_best(cnn, v)
+            hist.append(tuple(rec))
             log(msg)
     return hist
 
 
 def main():
+    """``python -m igm_amd.pretraining --out <file> [...]``.
+
+    --data patches: glacier_patch() draws of 128..512 cells plus crops of
+    multi-dome fields (the uneven-size batches of pretraining.py);
+    --data domes: square crops of whole 1000 x 1000 multi-dome fields drawn
+    with --field-seeds.  --val-seed S validates on the domes1000 field of
+    seed S against its converged float64 solution and keeps the best
+    weights; otherwise the last weights are saved."""
     from .config import default_cfg
     ap = argparse.ArgumentParser()
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--seconds", type=float, default=None)
     ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--data", choices=["patches", "domes"], default="patches")
     ap.add_argument("--pool", type=int, default=256)
+    ap.add_argument("--buckets", type=int, default=1)
+    ap.add_argument("--crop", type=int, default=512)
+    ap.add_argument("--field-seeds", type=int, nargs="*", default=list(range(100, 132)))
     ap.add_argument("--lr", type=float, default=1e-4)
     ap.add_argument("--lr-end", type=float, default=None)
     ap.add_argument("--scratch", action="store_true")
@@ -128,7 +219,11 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--crop-seeds", type=int, nargs="*", default=[10, 11, 12, 13])
     ap.add_argument("--crop-frac", type=float, default=0.5)
+    ap.add_argument("--val-seed", type=int, default=None)
+    ap.add_argument("--log-every", type=int, default=100)
     args = ap.parse_args()
+    if args.steps is None and args.seconds is None:
+        args.steps = 2000
     cfg = default_cfg()
     ice = cfg.processes.iceflow
     nz = int(ice.numerics.Nz)
@@ -139,14 +234,32 @@ def main():
         init = args.init or os.path.join(EM.EMULATOR_DIR, "pinnbp_10_4_cnn_16_32_2_1_a.safetensors")
         weights, _ = EM.load_weights(init)
     cnn = EM.CNN(nz, int(ice.emulator.network.nb_layers), device=dev, weights=weights, seed=args.seed)
-    # crops of 1000 x 1000 multi-dome fields drawn with other seeds than the
-    # benchmark field (seed 2), which stays out of the training data
-    crops = [syn.domes1000(seed=s) for s in args.crop_seeds]
-    pool = make_pool(args.pool, seed=args.seed, dev=dev, crops_from=crops, crop_frac=args.crop_frac)
-    train(cnn, pool, args.steps, dzeta, ice.physics, batch=args.batch, lr=args.lr, lr_end=args.lr_end,
-          seed=args.seed)
-    cnn.save(args.out, list(ice.emulator.fieldin), nz, float(ice.numerics.vert_spacing))
-    print("saved", args.out)
+    t0 = time.time()
+    if args.data == "domes":
+        fields = [patch_tensor(syn.domes1000(seed=s), dev) for s in args.field_seeds]
+        batcher = CropBatcher(fields, args.batch, args.crop, args.seed, dev)
+    else:
+        # crops of 1000 x 1000 multi-dome fields drawn with seeds of their own
+        crops = [syn.domes1000(seed=s) for s in args.crop_seeds]
+        pool = make_pool(args.pool, seed=args.seed, dev=dev, crops_from=crops, crop_frac=args.crop_frac)
+        batcher = Batcher(pool, args.batch, args.seed, dev, buckets=args.buckets)
+    print(f"data ready in {time.time() - t0:.1f}s", flush=True)
+    save = lambda m: m.save(args.out, list(ice.emulator.fieldin), nz, float(ice.numerics.vert_spacing))
+    val = on_best = None
+    if args.val_seed is not None:
+        t0 = time.time()
+        val = Validator(syn.domes1000(seed=args.val_seed), dzeta, ice.physics, dev)
+        print(f"validation field ready in {time.time() - t0:.1f}s (float64 stage change {val.change:.2e}); "
+              f"initial error {val(cnn):.4f}", flush=True)
+        on_best = lambda m, v: save(m)
+    hist = train(cnn, batcher, dzeta, ice.physics, steps=args.steps, seconds=args.seconds, lr=args.lr,
+                 lr_end=args.lr_end, log_every=args.log_every, val=val, on_best=on_best,
+                 log=lambda m: print(m, flush=True))
+    if val is None:
+        save(cnn)
+    else:
+        print(f"best validation error {min(h[2] for h in hist):.4f}", flush=True)
+    print("saved", args.out, flush=True)
 
 
 if __name__ == "__main__":

@@ -61,7 +61,8 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     dev = U0.device
     Ub = [U0.detach().clone().contiguous(), torch.empty_like(U0)]
     Vb = [V0.detach().clone().contiguous(), torch.empty_like(V0)]
-    mU, vU, mV, vV = (torch.zeros_like(Ub[0]) for _ in range(4))
+    # Adam slots, one (m_U, v_U, m_V, v_V) record per node and level
+    mom = torch.zeros(tuple(U0.shape) + (4,), dtype=dt, device=dev)
     es_host = L.IgmEarlyStop(best_loss=0.0, relative_min_delta=relative_min_delta, has_best=0,
                              wait=0, patience=patience if early_stopping else 2**30, stopped=0,
                              stop_iter=-1, iters_done=0)
@@ -78,8 +79,8 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
             optimizer.lr = lr_schedule(i)
         adam.alpha = optimizer.alpha(t0 + i + 1, dt)
         src, dst = i % 2, (i + 1) % 2
-        op.run_pass(2, Ub[src], Vb[src], Uo=Ub[dst], Vo=Vb[dst], mU=mU, vU=vU, mV=mV, vV=vV,
-                    adam=adam, stop_flag=stop_ptr)
+        op.run_pass(2, Ub[src], Vb[src], Uo=Ub[dst], Vo=Vb[dst], moments=mom, adam=adam,
+                    stop_flag=stop_ptr)
         op.finalize(es=es_ptr, cost_hist=hist, it=i)
         if early_stopping and (i + 1) % check_every == 0:
             if int(es_dev[L.EARLYSTOP_STOPPED_OFFSET:L.EARLYSTOP_STOPPED_OFFSET + 4]
@@ -92,3 +93,35 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     res = n % 2
     costs = hist[:n].cpu().numpy()
     return SolveResult(Ub[res], Vb[res], costs, n, bool(st.stopped))
+
+
+def adam_schedule(lr0, n_flat, n_decay, lr_end=1e-3):
+    """Step schedule: lr0 for n_flat iterations, then a geometric decay to
+    lr_end over n_decay iterations (held at lr_end afterwards)."""
+    d = (lr_end / lr0) ** (1.0 / max(1, n_decay))
+    return lambda i: lr0 if i < n_flat else max(lr_end, lr0 * d ** (i - n_flat))
+
+
+def rel_err(U, V, Ur, Vr, mask):
+    """Relative L2 velocity error ||(U,V) - (Ur,Vr)|| / ||(Ur,Vr)|| over the
+    cells where ``mask`` holds (all layers), in float64."""
+    d = torch.stack([U, V]).double() - torch.stack([Ur, Vr]).double()
+    r = torch.stack([Ur, Vr]).double()
+    m = mask.to(torch.float64)
+    return float(torch.linalg.vector_norm(d * m) / torch.linalg.vector_norm(r * m))
+
+
+def converged_solution(op32, op64, U0, V0, n_flat=8000, n_decay=20000, n_refine=2000):
+    """Float64 minimiser of one field: float32 Adam from (U0, V0) at lr 10
+    for n_flat iterations, a geometric decay to 1e-3 over n_decay, then
+    float64 Adam from that point with the step decaying 1e-3 -> 1e-5.
+
+    Returns (U, V, COST, change) with ``change`` the relative difference the
+    float64 stage made (a measure of how converged the float32 stage was).
+    """
+    r = solve(op32, U0, V0, n_flat + n_decay, LegacyAdamState(lr=1.0), early_stopping=False,
+              lr_schedule=adam_schedule(10.0, n_flat, n_decay))
+    r2 = solve(op64, r.U.double(), r.V.double(), n_refine, LegacyAdamState(lr=1.0), early_stopping=False,
+               lr_schedule=adam_schedule(1e-3, 0, n_refine, lr_end=1e-5))
+    change = rel_err(r.U, r.V, r2.U, r2.V, torch.ones_like(r.U[0], dtype=torch.bool))
+    return r2.U, r2.V, float(r2.costs[-1]), change

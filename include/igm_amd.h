@@ -97,24 +97,26 @@ int igm_energy_coeffs_f64(const double* thk, const double* usurf, const double* 
  *         solve/solve.py:178-180), per-block float64 partial sums.
  * mode 1: + gradient dJ/dU, dJ/dV into gU, gV (the GradientTape of
  *         solve/solve.py:198 and emulate/emulate.py:252).
- * mode 2: + legacy Adam update (solve/solve.py:205): reads U, V, m, v; writes
- *         Uout, Vout (must not alias U, V) and m, v in place.
+ * mode 2: + legacy Adam update (solve/solve.py:205): reads U, V and the
+ *         moments; writes Uout, Vout (must not alias U, V) and the moments in
+ *         place.  Planar velocity layout only.  moments: [nbatch][nz][Ny][Nx][4]
+ *         = (m_U, v_U, m_V, v_V) per node and level, contiguous (the state of
+ *         the reference's Adam slots, one 16-B record per node-level).
  * dzeta: host array of nz-1 layer thicknesses (vert_disc.py:26-35).
  * stop_flag (device, nullable): when *stop_flag != 0 the pass does nothing. */
 int igm_energy_pass_f32(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
                         const float* U, const float* V, int64_t bstride, int64_t kstride,
                         int64_t rstride, int64_t cstride, const float* coef, const float* hscale,
                         const double* dzeta, const IgmPhysics* phys, float* gU, float* gV,
-                        float* Uout, float* Vout, float* mU, float* vU, float* mV, float* vV,
-                        const IgmAdam* adam, double* partials, const int32_t* stop_flag,
-                        void* stream);
+                        float* Uout, float* Vout, float* moments, const IgmAdam* adam,
+                        double* partials, const int32_t* stop_flag, void* stream);
 int igm_energy_pass_f64(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
                         const double* U, const double* V, int64_t bstride, int64_t kstride,
                         int64_t rstride, int64_t cstride, const double* coef,
                         const double* hscale, const double* dzeta, const IgmPhysics* phys,
-                        double* gU, double* gV, double* Uout, double* Vout, double* mU,
-                        double* vU, double* mV, double* vV, const IgmAdam* adam,
-                        double* partials, const int32_t* stop_flag, void* stream);
+                        double* gU, double* gV, double* Uout, double* Vout, double* moments,
+                        const IgmAdam* adam, double* partials, const int32_t* stop_flag,
+                        void* stream);
 
 /* Deterministic float64 reduction of the pass's partial sums:
  * out[b] = {mean shear, mean sliding, mean gravity, COST} per batch entry
@@ -193,16 +195,20 @@ int64_t igm_wgrad_workspace_floats(int32_t taps, int32_t nblocks);
 
 /* Weight and bias gradient of a 3x3 (taps=9) or 1x1 (taps=1) layer on bf16
  * MFMA: dW_hwio [taps][32][cout] and db [cout] (fp32), summed over all
- * pixels of the batch in a fixed order (per-block partials + reduction). */
+ * pixels of the batch in a fixed order (per-block partials + reduction);
+ * accumulate != 0 adds to dW_hwio, db instead of overwriting them. */
 int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int32_t* extents,
                         int32_t B, int32_t H, int32_t W, int32_t cout, int32_t nblocks,
-                        float* workspace, float* dW_hwio, float* db, void* stream);
+                        float* workspace, int32_t accumulate, float* dW_hwio, float* db,
+                        void* stream);
 
-/* Weight and bias gradient of layer 1 (fp32): dW_hwio [3][3][5][32], db [32]. */
+/* Weight and bias gradient of layer 1 (fp32): dW_hwio [3][3][5][32], db [32]
+ * (accumulate != 0 adds to them). */
 int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const float* f3,
                       const float* f4, int64_t fb, int64_t fr, int64_t fc, const int32_t* extents,
                       int32_t B, int32_t H, int32_t W, const void* dz1, int32_t nblocks,
-                      float* workspace, float* dW_hwio, float* db, void* stream);
+                      float* workspace, int32_t accumulate, float* dW_hwio, float* db,
+                      void* stream);
 
 /* Legacy Keras Adam with per-tensor clipnorm on a flat fp32 parameter vector
  * (emulate.py:24-35, 267-269): tensor t spans [offs[t], offs[t+1]);

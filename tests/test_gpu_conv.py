@@ -73,7 +73,7 @@ def test_wgrad_3x3_and_1x1(dev, B, H, W):
     dW = torch.empty((3, 3, 32, 32), dtype=torch.float32, device=dev)
     db = torch.empty(32, dtype=torch.float32, device=dev)
     ad, dzd = a.to(torch.bfloat16).to(dev), dz.to(torch.bfloat16).to(dev)  # keep the buffers alive
-    L.call("igm_conv_wgrad_bf16", 9, L.ptr(ad), L.ptr(dzd), None, B, H, W, 32, 64, L.ptr(ws), L.ptr(dW),
+    L.call("igm_conv_wgrad_bf16", 9, L.ptr(ad), L.ptr(dzd), None, B, H, W, 32, 64, L.ptr(ws), 0, L.ptr(dW),
            L.ptr(db), L.stream_ptr())
     wt = torch.zeros((32, 32, 3, 3), dtype=torch.float64, requires_grad=True)
     y = F.conv2d(nchw(a).double(), wt, None, padding=1)
@@ -86,11 +86,14 @@ def test_wgrad_3x3_and_1x1(dev, B, H, W):
     dW1 = torch.empty((1, 1, 32, 20), dtype=torch.float32, device=dev)
     db1 = torch.empty(20, dtype=torch.float32, device=dev)
     dYd = dY.to(torch.bfloat16).to(dev)
-    L.call("igm_conv_wgrad_bf16", 1, L.ptr(ad), L.ptr(dYd), None, B, H, W, 20, 64, L.ptr(ws), L.ptr(dW1),
+    L.call("igm_conv_wgrad_bf16", 1, L.ptr(ad), L.ptr(dYd), None, B, H, W, 20, 64, L.ptr(ws), 0, L.ptr(dW1),
+           L.ptr(db1), L.stream_ptr())
+    # accumulate: a second call adds the same gradient
+    L.call("igm_conv_wgrad_bf16", 1, L.ptr(ad), L.ptr(dYd), None, B, H, W, 20, 64, L.ptr(ws), 1, L.ptr(dW1),
            L.ptr(db1), L.stream_ptr())
     ref = torch.einsum("bhwi,bhwo->io", a.double(), dY[..., :20].double())
-    assert rel(dW1.cpu()[0, 0], ref) <= 1e-5
-    assert rel(db1.cpu(), dY[..., :20].double().sum(dim=(0, 1, 2))) <= 1e-5
+    assert rel(dW1.cpu()[0, 0], 2 * ref) <= 1e-5
+    assert rel(db1.cpu(), 2 * dY[..., :20].double().sum(dim=(0, 1, 2))) <= 1e-5
 
 
 def test_conv_in_and_head(dev):
@@ -130,7 +133,7 @@ def test_conv_in_wgrad(dev):
     Xd = X.to(dev).contiguous()
     fields = EM.nhwc_fields(Xd)
     dzd = dz.to(torch.bfloat16).to(dev)
-    L.call("igm_conv_in_wgrad", *fields[0], *fields[1], None, B, H, W, L.ptr(dzd), 64, L.ptr(ws), L.ptr(dW),
+    L.call("igm_conv_in_wgrad", *fields[0], *fields[1], None, B, H, W, L.ptr(dzd), 64, L.ptr(ws), 0, L.ptr(dW),
            L.ptr(db), L.stream_ptr())
     wt = torch.zeros((32, 5, 3, 3), dtype=torch.float64, requires_grad=True)
     y = F.conv2d(nchw(X).double(), wt, None, padding=1)
@@ -244,3 +247,46 @@ def test_adam_clip(dev):
         alpha = 1e-3 * np.sqrt(1 - 0.999) / (1 - 0.9)
         w[a:e] -= (m * alpha) / (torch.sqrt(v) + 1e-7)
     assert rel(cnn.w.cpu(), w) <= 1e-6
+
+
+def test_bucketed_gradient_equals_padded(dev):
+    """Sub-batches with accumulated gradients give the gradient of the whole
+    zero-padded batch (the padding is masked, the sums only reorder)."""
+    from igm_amd.energy import EnergyOperator
+    from oracle.energy import DEFAULT_PHYSICS
+    from tests.helpers import dzeta_for
+    cnn = EM.CNN(nz=4, nb_layers=3, device=dev, seed=7)
+    dz = dzeta_for(4)
+    sizes = [(24, 30), (48, 64), (16, 90), (40, 40)]
+    parts = []
+    for b, (ny, nx) in enumerate(sizes):
+        g = small_glacier(ny, nx, seed=10 + b)
+        parts.append(torch.stack([torch.as_tensor(g[k]) for k in ("thk", "usurf", "arrhenius", "slidingco", "dX")],
+                                 -1).float().to(dev))
+
+    def pack(ps):
+        H = max(p.shape[0] for p in ps)
+        W = max(p.shape[1] for p in ps)
+        X = torch.zeros((len(ps), H, W, 5), device=dev)
+        for b, p in enumerate(ps):
+            X[b, :p.shape[0], :p.shape[1]] = p
+        ext = torch.tensor([list(p.shape[:2]) for p in ps], dtype=torch.int32, device=dev)
+        return X, ext
+
+    def grad(groups):
+        keep = []
+        for k, ps in enumerate(groups):
+            X, ext = pack(ps)
+            op = EnergyOperator(dz, DEFAULT_PHYSICS, X=X, extents=ext)
+            fields = EM.nhwc_fields(X)
+            Y, acts = cnn.forward(fields, extents=ext, keep=True)
+            e, gY = op.energy_grad(Y)
+            cnn.backward(gY, acts, fields, extents=ext, accumulate=k > 0)
+            keep.append((X, ext, fields, Y, acts, gY))
+        torch.cuda.synchronize()
+        return cnn.g.cpu().clone()
+
+    g_all = grad([parts])
+    g_bkt = grad([[parts[0], parts[3]], [parts[1], parts[2]]])
+    assert float(g_all.abs().max()) > 0
+    assert rel(g_bkt, g_all) <= 1e-5

@@ -152,7 +152,9 @@ def test_fused_adam_step(dev, dtype):
     m[3] += torch.as_tensor(rng.uniform(0, 1e-3, Ud.shape), dtype=dtype, device=dev)
     m_host = [t.cpu().double().clone() for t in m]
     adam = L.IgmAdam(alpha=st.alpha(3, dtype), beta1=0.9, beta2=0.999, epsilon=1e-7)
-    op.run_pass(2, Ud, Vd, Uo=Uo, Vo=Vo, mU=m[0], vU=m[1], mV=m[2], vV=m[3], adam=adam)
+    mom = torch.stack(m, -1).contiguous()  # (m_U, v_U, m_V, v_V) per node-level
+    op.run_pass(2, Ud, Vd, Uo=Uo, Vo=Vo, moments=mom, adam=adam)
+    m = [mom[..., i] for i in range(4)]
     # oracle
     _, _, gU, gV = oracle_eval(g, U, V, nz)
     opt = LegacyAdam(lr=1.0, iterations=2)
