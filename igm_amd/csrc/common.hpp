@@ -56,6 +56,20 @@ __device__ __forceinline__ double lane_from_left(double x) {
 }
 
 // Sum over the 64 lanes of a wavefront (every lane gets the total).
+// MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin
+// (workgroup p runs on XCD p % 8), and each XCD has an L2 of its own.
+// xcd_remap turns the dispatch index p of nb workgroups into a logical index
+// such that every XCD gets one contiguous range of logical indices: tiles
+// that are neighbours in the logical order then share an L2, and the cache
+// lines on their common edges are fetched from HBM once.  A bijection on
+// [0, nb) whatever the hardware order, so only speed depends on it.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_remap(int p, int nb) {
+  const int q = nb / kXcds, r = nb % kXcds;
+  const int x = p % kXcds, i = p / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, kWave);
@@ -77,6 +91,14 @@ __device__ __forceinline__ float pow_pos(float x, float e) {
   return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
 }
 __device__ __forceinline__ double pow_pos(double x, double e) { return pow(x, e); }
+// d * x^e for x, d > 0, with l2d = log2(d): one exp2 of e log2(x) + log2(d)
+// in fp32 (the multiply by d folds into the exponent)
+__device__ __forceinline__ float scaled_pow(float x, float e, float d, float l2d) {
+  return __builtin_amdgcn_exp2f(__builtin_fmaf(e, __builtin_amdgcn_logf(x), l2d));
+}
+__device__ __forceinline__ double scaled_pow(double x, double e, double d, double l2d) {
+  return d * pow(x, e);
+}
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 // hardware square root (v_sqrt_f32, ~1 ulp) -- not the correctly rounded expansion

@@ -114,9 +114,10 @@ conv_in_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
 // MODE 1 (dgrad): out = conv * (aux > 0 ? 1 : 0.01), aux = activation of the
 // layer below (its sign is the sign of its pre-activation).
 // Block = 4 waves; tile = 4 output rows x 64 pixels; wave w owns row y0 + w.
-// Persistent: block k walks tiles [k*per, (k+1)*per) of the tile list ordered
-// with the row-block index fastest, so a block's tiles are vertically
-// adjacent (their halo rows meet in the same L2) and the 18 weight
+// Persistent: logical block k walks tiles [k*per, (k+1)*per) of the tile
+// list ordered with the column-tile index fastest; logical blocks are laid
+// out XCD by XCD (xcd_remap), so each XCD sweeps a band of tile rows and the
+// halo rows and columns of neighbouring tiles meet in its L2.  The 18 weight
 // fragments are loaded once per block.
 template <int MODE>
 __global__ void __launch_bounds__(256)
@@ -129,7 +130,9 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
   const int nty = (H + kTileH - 1) / kTileH;
   const int ntx = (W + kTileW - 1) / kTileW;
   const int64_t ntiles = (int64_t)B * ntx * nty;
-  const int64_t t0 = (int64_t)blockIdx.x * per;
+  // XCD-contiguous runs of tiles: vertically adjacent tiles (ntx apart in
+  // the x-fastest order) then share one L2 for their halo rows
+  const int64_t t0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * per;
   if (t0 >= ntiles) return;
   const int64_t t1 = t0 + per < ntiles ? t0 + per : ntiles;
 
@@ -149,8 +152,8 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
   }
 
   for (int64_t ti = t0; ti < t1; ++ti) {
-    const int ty = (int)(ti % nty);
-    const int tx = (int)((ti / nty) % ntx);
+    const int tx = (int)(ti % ntx);
+    const int ty = (int)((ti / ntx) % nty);
     const int b = (int)(ti / ((int64_t)nty * ntx));
     const int x0 = tx * kTileW, y0 = ty * kTileH;
     int ny, nx;

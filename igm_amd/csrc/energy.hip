@@ -204,6 +204,7 @@ struct PassArgs {
   uint32_t bs_bytes;    // batch stride of the velocity layout, bytes
   int Ny, Nx, plane, nstrips, nrowblk, blocks_per_batch;
   T dz[IGM_MAX_NZ];   // layer thickness dzeta_l
+  T l2dz[IGM_MAX_NZ]; // log2(dzeta_l)
   T idz[IGM_MAX_NZ];  // 1/dzeta_l
   T wz[IGM_MAX_NZ];   // level weight of the gravity term: (dzeta_{k-1} + dzeta_k)/2
   T e_glen;           // (p-2)/2
@@ -214,13 +215,75 @@ struct PassArgs {
   T alpha, omb1, omb2, eps;  // Adam: alpha, 1-beta1, 1-beta2, epsilon
 };
 
+// ------------------------------------------------------------- finalize --
+//
+// Sum of one batch entry's per-block partials in a fixed order (strided
+// per-thread sums, a wave butterfly, then the 4 waves in order) into
+// out[4] = {mean shear, mean sliding, mean gravity, COST}, then the
+// EarlyStopping rule (iceflow/utils.py:186-204) when es is given.  Run by one
+// 256-thread block.  (It stays a kernel of its own: folding it into the pass
+// as "last block sums" needs a device-scope release per block, which on the
+// multi-XCD MI355X writes back the whole L2 -- measured 18 us per iteration
+// against 7 us for this launch.)
+__device__ void finalize_entry(const double* p, int nblk, double* out, IgmEarlyStop* es,
+                               double* cost_hist, int iter) {
+  __shared__ double fin[4][3];
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int i = threadIdx.x; i < nblk; i += 256) {
+    s0 += p[3 * i];
+    s1 += p[3 * i + 1];
+    s2 += p[3 * i + 2];
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if ((threadIdx.x & 63) == 0) {
+    fin[threadIdx.x >> 6][0] = s0;
+    fin[threadIdx.x >> 6][1] = s1;
+    fin[threadIdx.x >> 6][2] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double sh = ((fin[0][0] + fin[1][0]) + fin[2][0]) + fin[3][0];
+    const double sl = ((fin[0][1] + fin[1][1]) + fin[2][1]) + fin[3][1];
+    const double gr = ((fin[0][2] + fin[1][2]) + fin[2][2]) + fin[3][2];
+    const double cost = (sh + sl) + gr;  // tf.add_n in energy_components order
+    out[0] = sh;
+    out[1] = sl;
+    out[2] = gr;
+    out[3] = cost;
+    if (es != nullptr) {
+      if (cost_hist != nullptr) cost_hist[iter] = cost;
+      es->iters_done = iter + 1;
+      if (!es->has_best) {
+        es->best_loss = cost;
+        es->has_best = 1;
+      } else {
+        const double rel = (es->best_loss - cost) / fabs(es->best_loss);
+        if (rel > es->relative_min_delta) {
+          es->best_loss = cost;
+          es->wait = 0;
+        } else {
+          es->wait += 1;
+          if (es->wait >= es->patience) {
+            es->stopped = 1;
+            es->stop_iter = iter;
+          }
+        }
+      }
+    }
+  }
+}
+
 // INTER = 0: planar (Nz, Ny, Nx) with separate U, V buffers;
 // INTER = 1: node-interleaved Y (Ny, Nx, 2Nz), V = Y + Nz.
 template <typename T, int NZ, int MODE, int INTER>
 __global__ void __launch_bounds__(kPassBlock)
 energy_pass_kernel(const PassArgs<T> a) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  // logical block: XCD-contiguous bands of (strip, row block) tiles
+  const int lblk = xcd_remap(blockIdx.x, gridDim.x);
+  const int wid = lblk * kWavesPerBlock + (threadIdx.x >> 6);
   const int b = blockIdx.y;
   double e_sh = 0.0, e_sl = 0.0, e_gr = 0.0;
   const bool stopped = (a.stop != nullptr) && (*a.stop != 0);
@@ -242,6 +305,7 @@ energy_pass_kernel(const PassArgs<T> a) {
     const uint32_t kstep = INTER ? ES : (uint32_t)a.plane * ES;   // bytes between levels
     const T hh = a.hscale[b];
     const T hh2 = hh * hh;
+    const T h05 = T(0.5) * hh2, h025 = T(0.25) * hh2;
 
     // byte offset of node (row, ic), level 0 -- or kOOB outside the grid
     auto node_off = [&](int row, bool extra) -> uint32_t {
@@ -306,6 +370,7 @@ energy_pass_kernel(const PassArgs<T> a) {
       // keep the loads above ahead of the sweep (the scheduler would sink them)
       __builtin_amdgcn_sched_barrier(0);
       const T q = cc[0], w = cc[1], gx = cc[2], gy = cc[3], csl = cc[4], bx = cc[5], by = cc[6];
+      const T whh = w * hh2;
 
       // legacy-Adam update of node (jc, ic), level k, from its gradient
       auto adam = [&](int k, T g_u, T g_v, T (&m)[4]) {
@@ -388,22 +453,26 @@ energy_pass_kernel(const PassArgs<T> a) {
           const T dUdz = (sU - p_su) * rz, dVdz = (sV - p_sv) * rz;
           const T Dux = dUdx - dUdz * bx, Duy = dUdy - dUdz * by;
           const T Dvx = dVdx - dVdz * bx, Dvy = dVdy - dVdz * by;
-          const T Exy = T(0.5) * (Dvx + Duy);
+          const T P = Dvx + Duy;  // 2 Exy
           const T Ezz = -(Dux + Dvy);
-          const T sr2 = hh2 * (T(0.5) * (Dux * Dux + T(2) * (Exy * Exy) + Dvy * Dvy + Ezz * Ezz) +
-                               T(0.25) * (dUdz * dUdz + dVdz * dVdz));
+          // sr2 = hh2 (0.5 (Dux^2 + Dvy^2 + Ezz^2) + Exy^2 + 0.25 (dUdz^2 + dVdz^2))
+          const T t1 = (Dux * Dux + Dvy * Dvy) + Ezz * Ezz;
+          const T t2 = (P * P + dUdz * dUdz) + dVdz * dVdz;
+          const T sr2 = h05 * t1 + h025 * t2;
           const T sclip = clamp_(sr2, a.min_sr2, a.max_sr2);
           const bool in_rng = (sclip == sr2);  // the clip passes its gradient inside the range
           const T sc = sclip + a.regu2;
-          const T pw = pow_pos(sc, a.e_glen);
-          const T wl = a.dz[l] * pw;
+          const T wl = scaled_pow(sc, a.e_glen, a.dz[l], a.l2dz[l]);  // dzeta_l sc^e
           c_sh += wl * sr2;
-          // adjoint of the layer (in the scaled variables)
-          const T ratio = in_rng ? a.e_glen * sr2 * rcp(sc) : T(0);
-          const T G = w * wl * (T(1) + ratio) * hh2;
-          const T gDux = G * (Dux - Ezz), gDvy = G * (Dvy - Ezz), gDxy = G * Exy;
-          const T gdUdz = T(0.5) * G * dUdz - gDux * bx - gDxy * by;
-          const T gdVdz = T(0.5) * G * dVdz - gDxy * bx - gDvy * by;
+          // adjoint of the layer (in the scaled variables):
+          // G = w hh2 wl (1 + e sr2 / sc), zero ratio outside the clip range
+          const T WL = whh * wl;
+          const T er = in_rng ? a.e_glen * sr2 : T(0);
+          const T G = WL + WL * (er * rcp(sc));
+          const T Gh = T(0.5) * G;
+          const T gDux = G * (Dux - Ezz), gDvy = G * (Dvy - Ezz), gDxy = Gh * P;
+          const T gdUdz = Gh * dUdz - gDux * bx - gDxy * by;
+          const T gdVdz = Gh * dVdz - gDxy * bx - gDvy * by;
           g_ux += gDux;
           g_uy += gDxy;
           g_vx += gDxy;
@@ -474,7 +543,7 @@ energy_pass_kernel(const PassArgs<T> a) {
       s1 += red[wv][1];
       s2 += red[wv][2];
     }
-    double* o = a.partials + 3 * ((int64_t)b * a.blocks_per_batch + blockIdx.x);
+    double* o = a.partials + 3 * ((int64_t)b * a.blocks_per_batch + lblk);
     o[0] = s0;
     o[1] = s1;
     o[2] = s2;
@@ -522,54 +591,8 @@ energy_finalize_kernel(const double* __restrict__ partials, int blocks_per_batch
                        double* __restrict__ out, IgmEarlyStop* es, double* cost_hist, int iter) {
   const int b = blockIdx.x;
   if (es != nullptr && es->stopped) return;
-  __shared__ double red[kFinBlock][3];
-  double s0 = 0, s1 = 0, s2 = 0;
-  const double* p = partials + 3 * (int64_t)b * blocks_per_batch;
-  for (int i = threadIdx.x; i < blocks_per_batch; i += kFinBlock) {
-    s0 += p[3 * i];
-    s1 += p[3 * i + 1];
-    s2 += p[3 * i + 2];
-  }
-  red[threadIdx.x][0] = s0;
-  red[threadIdx.x][1] = s1;
-  red[threadIdx.x][2] = s2;
-  __syncthreads();
-  for (int h = kFinBlock / 2; h > 0; h >>= 1) {
-    if (threadIdx.x < h) {
-      red[threadIdx.x][0] += red[threadIdx.x + h][0];
-      red[threadIdx.x][1] += red[threadIdx.x + h][1];
-      red[threadIdx.x][2] += red[threadIdx.x + h][2];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double sh = red[0][0], sl = red[0][1], gr = red[0][2];
-    const double cost = (sh + sl) + gr;  // tf.add_n in energy_components order
-    out[4 * b + 0] = sh;
-    out[4 * b + 1] = sl;
-    out[4 * b + 2] = gr;
-    out[4 * b + 3] = cost;
-    if (es != nullptr) {
-      if (cost_hist != nullptr) cost_hist[iter] = cost;
-      es->iters_done = iter + 1;
-      if (!es->has_best) {
-        es->best_loss = cost;
-        es->has_best = 1;
-      } else {
-        const double rel = (es->best_loss - cost) / fabs(es->best_loss);
-        if (rel > es->relative_min_delta) {
-          es->best_loss = cost;
-          es->wait = 0;
-        } else {
-          es->wait += 1;
-          if (es->wait >= es->patience) {
-            es->stopped = 1;
-            es->stop_iter = iter;
-          }
-        }
-      }
-    }
-  }
+  finalize_entry(partials + 3 * (int64_t)b * blocks_per_batch, blocks_per_batch, out + 4 * b, es,
+                 cost_hist, iter);
 }
 
 // ------------------------------------------------------ velocity outputs --
@@ -665,11 +688,12 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
   a.Ny = Ny; a.Nx = Nx;
   const PassGeom g = pass_geom(nbatch, Ny, Nx);
   a.nstrips = g.nstrips; a.nrowblk = g.nrowblk; a.blocks_per_batch = g.blocks;
-  for (int k = 0; k < IGM_MAX_NZ; ++k) { a.dz[k] = a.idz[k] = a.wz[k] = T(0); }
+  for (int k = 0; k < IGM_MAX_NZ; ++k) { a.dz[k] = a.idz[k] = a.wz[k] = a.l2dz[k] = T(0); }
   for (int l = 0; l < nz - 1; ++l) {
     IGM_CHECK_ARG(dzeta[l] > 0.0, "dzeta must be positive");
     a.dz[l] = T(dzeta[l]);
     a.idz[l] = T(1.0 / dzeta[l]);
+    a.l2dz[l] = T(std::log2(dzeta[l]));
   }
   for (int k = 0; k < nz; ++k) {
     const double lo = k > 0 ? dzeta[k - 1] : 0.0;

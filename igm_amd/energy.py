@@ -93,7 +93,7 @@ class EnergyOperator:
         self.coef = torch.empty((B, 7, Ny, Nx), dtype=dtype, device=dev)
         self.hscale = torch.empty((B,), dtype=dtype, device=dev)
         nws = L.load().igm_energy_workspace_doubles(B, Ny, Nx)
-        self.partials = torch.empty((max(1, nws),), dtype=torch.float64, device=dev)
+        self.partials = torch.zeros((max(1, nws),), dtype=torch.float64, device=dev)
         self.out = torch.zeros((B, 4), dtype=torch.float64, device=dev)
         L.call(f"igm_energy_coeffs_{self.sfx}", *ptrs, fb, fr, fc, L.ptr(extents), B, Ny, Nx,
                C.addressof(self.phys), L.ptr(self.coef), L.ptr(self.hscale), L.stream_ptr())

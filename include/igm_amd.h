@@ -103,7 +103,8 @@ int igm_energy_coeffs_f64(const double* thk, const double* usurf, const double* 
  *         = (m_U, v_U, m_V, v_V) per node and level, contiguous (the state of
  *         the reference's Adam slots, one 16-B record per node-level).
  * dzeta: host array of nz-1 layer thicknesses (vert_disc.py:26-35).
- * stop_flag (device, nullable): when *stop_flag != 0 the pass does nothing. */
+ * stop_flag (device, nullable): when *stop_flag != 0 the pass does nothing.
+ * partials: igm_energy_workspace_doubles() doubles (per-block sums). */
 int igm_energy_pass_f32(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
                         const float* U, const float* V, int64_t bstride, int64_t kstride,
                         int64_t rstride, int64_t cstride, const float* coef, const float* hscale,
