@@ -41,8 +41,14 @@ BENCH_WEIGHTS = ("igm_amd_cnn_16_32_nz10.safetensors", "pinnbp_10_4_cnn_16_32_2_
 # the fused Adam iteration U, V, m, v of both read (240 + 16), written (240)
 BYTES_ENERGY_ADJOINT = 176.0
 BYTES_FUSED_ITERATION = 496.0
+# transport: ubar, vbar, thk, smb, topg read (20), thk, usurf written (8)
+BYTES_TRANSPORT = 28.0
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+# `bench.py --kernels-only`, scaled by a 1 GiB copy (the roofline's `traffic`)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_domes1000.json")
 # schedule family searched for both paths: initial step, then total iterations
 LR0S = (3.0, 10.0, 30.0)
+CPU_SAMPLE_S = 15.0  # wall time of the CPU solver-iteration sample
 REFINE_TOTALS = (100, 200, 300, 500, 700, 1000, 1500, 2000, 3000, 4500, 7000, 10000, 14000)
 SOLVER_TOTALS = (1000, 1500, 2000, 3000, 4500, 7000, 10000, 14000, 20000)
 
@@ -82,6 +88,77 @@ def reference_solution(f32, f64, thk, nz, dzeta, n_flat=8000, n_decay=20000, n_r
     return U, V, J, change, op64
 
 
+def measured_traffic(kernel):
+    """Counter-measured HBM bytes per launch of `kernel`, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return float(json.load(f)["kernels"][kernel]["traffic_bytes"])
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
+def measure_kernels(cnn, op32, U, V, f32, vert_weight, nz, nlaunch=50):
+    """Each hot kernel alone, back to back on the launch stream, timed with
+    HIP events at the workload size: the 3x3 bf16 MFMA conv, the energy +
+    adjoint pass, the fused Adam pass, the upwind transport, and a 1 GiB
+    device copy for scale.  (U, V): a realistic velocity field."""
+    from igm_amd import _lib as L
+    from igm_amd.fields import velocity_outputs
+    thk = f32["thk"]
+    Ny, Nx = thk.shape
+    dev = thk.device
+    st = L.stream_ptr()
+    a_in = torch.zeros((1, Ny, Nx, 32), dtype=torch.bfloat16, device=dev)
+    a_in.copy_(torch.randn((1, Ny, Nx, 32), device=dev).to(torch.bfloat16))
+    a_out = torch.empty_like(a_in)
+
+    def timed(fn):
+        for _ in range(5):
+            fn()
+        with Timer() as tm:
+            for _ in range(nlaunch):
+                fn()
+        return tm.ms * 1e3 / nlaunch
+
+    conv_us = timed(lambda: L.call("igm_conv3x3_fwd_bf16", L.ptr(a_in), L.ptr(cnn.fwd[0]), L.ptr(cnn.bias(1)),
+                                   None, 1, Ny, Nx, L.ptr(a_out), st))
+    conv_tflops = 2.0 * 9 * 32 * 32 * Ny * Nx / (conv_us * 1e-6) / 1e12
+    gU, gV = torch.empty_like(U), torch.empty_like(V)
+    en_us = timed(lambda: op32.run_pass(1, U, V, gU=gU, gV=gV))
+    mom = torch.zeros((nz, Ny, Nx, 4), dtype=torch.float32, device=dev)
+    Uo, Vo = torch.empty_like(U), torch.empty_like(V)
+    adam = L.IgmAdam(alpha=1e-3, beta1=0.9, beta2=0.999, epsilon=1e-7)
+    fused_us = timed(lambda: op32.run_pass(2, U, V, Uo=Uo, Vo=Vo, moments=mom, adam=adam))
+    # the upwind thickness update (superbee) on these velocities
+    vo = velocity_outputs(U, V, thk, vert_weight, nz)
+    smb = torch.zeros_like(thk)
+    topg = (f32["usurf"] - thk).contiguous()
+    thk_o, us_o = torch.empty_like(thk), torch.empty_like(thk)
+    tr_us = timed(lambda: L.call("igm_transport_f32", Ny, Nx, L.ptr(vo["ubar"]), L.ptr(vo["vbar"]), L.ptr(thk),
+                                 L.ptr(smb), L.ptr(topg), 50.0, 50.0, 0.01, 2, 0.91, 0.0, L.ptr(thk_o), None,
+                                 L.ptr(us_o), None, st))
+    # a plain device copy of 1 GiB, for scale (what HBM gives a streaming kernel)
+    ca = torch.empty(2 ** 28, dtype=torch.float32, device=dev).fill_(1.0)
+    cb = torch.empty_like(ca)
+    for _ in range(3):
+        cb.copy_(ca)
+    with Timer() as tm:
+        for _ in range(10):
+            cb.copy_(ca)
+    copy_gbs = 2 * ca.numel() * 4 * 10 / (tm.ms * 1e-3) / 1e9
+    del ca, cb
+    cells = Ny * Nx
+    r = {"conv_us": conv_us, "conv_tflops": conv_tflops,
+         "energy_adjoint_us": en_us, "energy_adjoint_gbs": BYTES_ENERGY_ADJOINT * cells / (en_us * 1e-6) / 1e9,
+         "fused_us": fused_us, "fused_gbs": BYTES_FUSED_ITERATION * cells / (fused_us * 1e-6) / 1e9,
+         "transport_us": tr_us, "transport_gbs": BYTES_TRANSPORT * cells / (tr_us * 1e-6) / 1e9,
+         "copy_gbs": copy_gbs}
+    log(f"[bench] conv3x3 {conv_us:.1f} us/launch = {conv_tflops:.1f} TFLOP/s; energy+adjoint {en_us:.1f} us = "
+        f"{r['energy_adjoint_gbs']:.0f} GB/s; fused Adam pass {fused_us:.1f} us = {r['fused_gbs']:.0f} GB/s; "
+        f"transport {tr_us:.1f} us = {r['transport_gbs']:.0f} GB/s; device copy {copy_gbs:.0f} GB/s")
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +170,8 @@ def main():
     ap.add_argument("--check-its", type=int, default=None, help="fallback check iterations (cfg default if unset)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--weights", default=None)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time the hot kernels alone on the emulated field and exit (a short run for counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the ice-flow path runs on a single MI355X "
@@ -146,6 +225,14 @@ def main():
     dzeta = np.asarray(state.dzeta)
     log(f"[bench] setup {time.time() - t0:.1f}s; weights {os.path.basename(wpath)}")
 
+    if args.kernels_only:
+        op32 = EnergyOperator(dzeta, DEFAULT_PHYSICS, fields=f32)
+        EM.update_iceflow_emulated(cfg, state)
+        kt = measure_kernels(state.iceflow_model, op32, state.U.contiguous(), state.V.contiguous(), f32,
+                             state.vert_weight, nz, nlaunch=10)
+        print(json.dumps({"kernels_us": {k: round(v, 2) for k, v in kt.items() if k.endswith("_us")}}), flush=True)
+        return
+
     # ------------------------------------------------ float64 minimiser --
     t = time.time()
     Ur, Vr, J64, ref_change, op64 = reference_solution(f32, f64, thk, nz, dzeta, args.ref_flat, args.ref_decay)
@@ -158,23 +245,42 @@ def main():
 
     def schedule_search(U0, V0, totals):
         """Shortest schedule of the family {lr0 in LR0S; 30 % flat, then a
-        geometric decay to 1e-3} from (U0, V0) that ends <= 1 %: returns
-        (its, lr0, err) of the first total that does (or of the last tried)."""
-        best = None
-        for total in totals:
+        geometric decay to 1e-3} from (U0, V0) that ends <= 1 %: the first
+        total of ``totals`` that does, then bisected (3 steps, to 50
+        iterations) against the total before it.  Returns (its, lr0, err)
+        (the last tried if none reaches 1 %)."""
+
+        def trial(total):
             if total == 0:
-                cand = [(0, None, rel_err(U0, V0, Ur, Vr, mask))]
-            else:
-                cand = []
-                for lr0 in LR0S:
-                    n_flat = int(0.3 * total)
-                    r = solve(op32, U0, V0, total, LegacyAdamState(lr=lr0), early_stopping=False,
-                              lr_schedule=adam_schedule(lr0, n_flat, total - n_flat))
-                    cand.append((total, lr0, rel_err(r.U, r.V, Ur, Vr, mask)))
+                return (0, None, rel_err(U0, V0, Ur, Vr, mask))
+            cand = []
+            for lr0 in LR0S:
+                n_flat = int(0.3 * total)
+                r = solve(op32, U0, V0, total, LegacyAdamState(lr=lr0), early_stopping=False,
+                          lr_schedule=adam_schedule(lr0, n_flat, total - n_flat))
+                cand.append((total, lr0, rel_err(r.U, r.V, Ur, Vr, mask)))
             best = min(cand, key=lambda c: c[2])
             log(f"[bench]   {total} its: best rel err {best[2]:.4f} (lr0 {best[1]})")
+            return best
+
+        lo, best = None, None
+        for total in totals:
+            best = trial(total)
             if best[2] <= 0.01:
                 break
+            lo = total
+        if best[2] > 0.01 or lo is None:
+            return best
+        hi = best[0]
+        for _ in range(3):
+            mid = int(round((lo + hi) / 100.0)) * 50
+            if mid <= lo or mid >= hi:
+                break
+            t = trial(mid)
+            if t[2] <= 0.01:
+                hi, best = mid, t
+            else:
+                lo = mid
         return best
 
     # ------------------------------------------------- emulator path --
@@ -183,6 +289,9 @@ def main():
     EM.update_iceflow_emulated(cfg, state)
     U_emu, V_emu = state.U.clone(), state.V.clone()
     inf_err = rel_err(U_emu, V_emu, Ur, Vr, mask)
+    # the energy gap the fallback rule looks at, against the float64 minimiser
+    j_emu = float(op32.energy(U_emu, V_emu)[0, 3])
+    gap_emu = (j_emu - J64) / abs(J64)
     with Timer() as tm:
         for _ in range(args.steps):
             EM.update_iceflow_emulated(cfg, state)
@@ -245,90 +354,80 @@ def main():
     es_err = rel_err(res_es.U, res_es.V, Ur, Vr, mask)
 
     # ------------------------------------------------------- rooflines --
-    cnn = state.iceflow_model
-    fields = EM.planar_fields([state.thk, state.usurf, state.arrhenius, state.slidingco, state.dX])
-    a_in = torch.zeros((1, Ny, Nx, 32), dtype=torch.bfloat16, device=dev)
-    a_in.copy_(torch.randn((1, Ny, Nx, 32), device=dev).to(torch.bfloat16))
-    a_out = torch.empty_like(a_in)
-    nlaunch = 50
-    st = L.stream_ptr()
-    for _ in range(5):
-        L.call("igm_conv3x3_fwd_bf16", L.ptr(a_in), L.ptr(cnn.fwd[0]), L.ptr(cnn.bias(1)), None, 1, Ny, Nx,
-               L.ptr(a_out), st)
-    with Timer() as tm:
-        for _ in range(nlaunch):
-            L.call("igm_conv3x3_fwd_bf16", L.ptr(a_in), L.ptr(cnn.fwd[0]), L.ptr(cnn.bias(1)), None, 1, Ny, Nx,
-                   L.ptr(a_out), st)
-    conv_us = tm.ms * 1e3 / nlaunch
-    conv_flop = 2.0 * 9 * 32 * 32 * Ny * Nx
-    conv_tflops = conv_flop / (conv_us * 1e-6) / 1e12
-    # energy + adjoint (mode 1) and the fused solver iteration (mode 2), each
-    # kernel alone, back to back on the launch stream
-    gU, gV = torch.empty_like(z), torch.empty_like(z)
-    Uw, Vw = Ur.float().contiguous(), Vr.float().contiguous()
-    for _ in range(5):
-        op32.run_pass(1, Uw, Vw, gU=gU, gV=gV)
-    with Timer() as tm:
-        for _ in range(nlaunch):
-            op32.run_pass(1, Uw, Vw, gU=gU, gV=gV)
-    en_us = tm.ms * 1e3 / nlaunch
-    en_gbs = BYTES_ENERGY_ADJOINT * Ny * Nx / (en_us * 1e-6) / 1e9
-    mom = torch.zeros((nz, Ny, Nx, 4), dtype=torch.float32, device=dev)
-    Uo, Vo = torch.empty_like(z), torch.empty_like(z)
-    adam = L.IgmAdam(alpha=1e-3, beta1=0.9, beta2=0.999, epsilon=1e-7)
-    for _ in range(5):
-        op32.run_pass(2, Uw, Vw, Uo=Uo, Vo=Vo, moments=mom, adam=adam)
-    with Timer() as tm:
-        for _ in range(nlaunch):
-            op32.run_pass(2, Uw, Vw, Uo=Uo, Vo=Vo, moments=mom, adam=adam)
-    fused_us = tm.ms * 1e3 / nlaunch
-    fused_gbs = BYTES_FUSED_ITERATION * Ny * Nx / (fused_us * 1e-6) / 1e9
-    # a plain device copy of 1 GiB, for scale (what HBM gives a streaming kernel)
-    ca = torch.empty(2 ** 28, dtype=torch.float32, device=dev).fill_(1.0)
-    cb = torch.empty_like(ca)
-    for _ in range(3):
-        cb.copy_(ca)
-    with Timer() as tm:
-        for _ in range(10):
-            cb.copy_(ca)
-    copy_gbs = 2 * ca.numel() * 4 * 10 / (tm.ms * 1e-3) / 1e9
-    del ca, cb
-    log(f"[bench] conv3x3 {conv_us:.1f} us/launch = {conv_tflops:.1f} TFLOP/s; energy+adjoint {en_us:.1f} us = "
-        f"{en_gbs:.0f} GB/s; fused Adam pass {fused_us:.1f} us = {fused_gbs:.0f} GB/s; solver iteration "
-        f"(pass + finalize + gaps) {it_ms * 1e3:.1f} us; device copy {copy_gbs:.0f} GB/s")
+    kt = measure_kernels(state.iceflow_model, op32, Ur.float().contiguous(), Vr.float().contiguous(), f32,
+                         state.vert_weight, nz)
+    conv_us, conv_tflops = kt["conv_us"], kt["conv_tflops"]
+    en_us, en_gbs = kt["energy_adjoint_us"], kt["energy_adjoint_gbs"]
+    fused_us, fused_gbs = kt["fused_us"], kt["fused_gbs"]
+    tr_us, tr_gbs, copy_gbs = kt["transport_us"], kt["transport_gbs"], kt["copy_gbs"]
+    log(f"[bench] solver iteration (pass + finalize + gaps) {it_ms * 1e3:.1f} us; raw emulator energy gap "
+        f"{gap_emu:.4%}")
 
     # the dominant kernel of the timed step: the fused Adam energy pass when
     # the field is polished, the 3x3 conv otherwise
     if n_ref > 0:
+        traffic = measured_traffic("energy_pass_kernel<float,10,2,0> (fused energy + adjoint + Adam)")
         roofline = {"bound": "hbm", "achieved": round(fused_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(fused_gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                    "frac": round(fused_gbs / PEAK_HBM_GBS, 4),
+                    "traffic": round(traffic) if traffic else None,
+                    "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                     "kernel": "energy_pass_kernel<float, 10, 2, 0> (fused energy + adjoint + Adam, 1000x1000)",
                     "bytes_per_cell": BYTES_FUSED_ITERATION, "us_per_launch": round(fused_us, 2),
                     "launches_per_step": int(n_ref)}
     else:
+        traffic = measured_traffic("conv3x3_mfma_kernel<0> (32->32 bf16)")
         roofline = {"bound": "mfma", "achieved": round(conv_tflops, 2), "peak": PEAK_BF16_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(conv_tflops / PEAK_BF16_TFLOPS, 4), "traffic": None,
+                    "unit": "TFLOP/s", "frac": round(conv_tflops / PEAK_BF16_TFLOPS, 4),
+                    "traffic": round(traffic) if traffic else None,
                     "kernel": "conv3x3_mfma_kernel<0> (32->32, 1000x1000)", "us_per_launch": round(conv_us, 2),
                     "launches_per_step": 15}
 
     # ----------------------------------------------------- CPU baseline --
+    # the oracle (torch-CPU float32 restatement of the reference's TF path) on
+    # this host: one emulator inference and a bounded number of solver
+    # iterations (energy + autograd gradient + legacy Adam) on the same field,
+    # scaled to the iteration counts the GPU paths need for <= 1 %
     cpu = None
     if not args.no_cpu:
         from oracle import cnn as OC
+        from oracle import energy as OE
+        from oracle.solver import LegacyAdam
         ncore = os.cpu_count() or 1
         torch.set_num_threads(min(16, ncore))
         X = torch.stack([torch.as_tensor(g[k]) for k in keys], -1)[None]
-        weights = cnn.get_weights()
-        OC.forward(X[:, :64, :64], weights)
-        tc = time.time()
+        weights = state.iceflow_model.get_weights()
         with torch.no_grad():
+            OC.forward(X[:, :64, :64], weights)
+            tc = time.time()
             OC.forward(X, weights)
         cpu_inf_s = time.time() - tc
-        cpu = {"value": 1.0 / cpu_inf_s, "unit": "velocity fields/s", "cores": torch.get_num_threads(),
+        fin = OE.fieldin_tensors(*(g[k] for k in keys))
+        Uc = torch.zeros((nz, Ny, Nx))
+        Vc = torch.zeros_like(Uc)
+        mvs = [torch.zeros_like(Uc) for _ in range(4)]
+        opt = LegacyAdam(lr=float(lr1))
+
+        def cpu_iteration():
+            _, _, gU_, gV_ = OE.energy_and_grad(Uc, Vc, fin, dzeta, DEFAULT_PHYSICS)
+            opt.apply([Uc, Vc], [gU_, gV_], [mvs[0], mvs[2]], [mvs[1], mvs[3]])
+
+        cpu_iteration()  # warm-up
+        tc, k = time.time(), 0
+        while k < 8 and (k < 2 or time.time() - tc < CPU_SAMPLE_S):
+            cpu_iteration()
+            k += 1
+        cpu_it_s = (time.time() - tc) / k
+        cpu_emu_s = cpu_inf_s + n_ref * cpu_it_s
+        cpu = {"value": round(1.0 / cpu_emu_s, 8), "unit": "velocity fields/s", "cores": torch.get_num_threads(),
                "kind": "port",
-               "sample": "one full 1000x1000 emulator inference (17 conv layers, fp32) of the oracle CNN "
-                         "(torch-CPU restatement of emulate/neural_network.py); TensorFlow is not installed"}
-        log(f"[bench] CPU oracle CNN inference {cpu_inf_s:.2f}s on {torch.get_num_threads()} threads")
+               "sample": f"oracle emulator inference of the 1000x1000 field ({cpu_inf_s:.2f} s) + {k} oracle solver "
+                         f"iterations ({cpu_it_s:.2f} s each), scaled to the {n_ref} polish iterations of the "
+                         "GPU emulated path; TensorFlow is not installed",
+               "solver_fields_per_s": round(1.0 / (n1 * cpu_it_s), 8),
+               "emulator_inference_fields_per_s": round(1.0 / cpu_inf_s, 4),
+               "iteration_s": round(cpu_it_s, 4)}
+        log(f"[bench] CPU oracle: CNN inference {cpu_inf_s:.2f} s, solver iteration {cpu_it_s:.2f} s "
+            f"({k} timed) on {torch.get_num_threads()} threads")
 
     out = {
         "metric": METRIC,
@@ -354,7 +453,7 @@ def main():
         "cpu_baseline": cpu,
         "emulated_rel_err_vs_fp64": round(emu_err, 5),
         "emulator_inference": {"ms": round(inf_ms, 4), "fields_per_s": round(1000.0 / inf_ms, 1),
-                               "rel_err_vs_fp64": round(inf_err, 5)},
+                               "rel_err_vs_fp64": round(inf_err, 5), "energy_gap_vs_fp64": round(gap_emu, 6)},
         "fallbacks": int(state.fallback_count),
         "solver_fields_per_s": round(solver_fps, 4),
         "solver_schedule": {"lr0": lr1, "flat": int(n_flat1), "decay_to_1e-3": int(n1 - n_flat1),
@@ -372,6 +471,9 @@ def main():
                              "unit": "TFLOP/s", "frac": round(conv_tflops / PEAK_BF16_TFLOPS, 4),
                              "us_per_launch": round(conv_us, 2),
                              "hbm_gbs": round(2 * 64.0 * Ny * Nx / (conv_us * 1e-6) / 1e9, 1)},
+        "roofline_transport": {"bound": "hbm", "achieved": round(tr_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                               "frac": round(tr_gbs / PEAK_HBM_GBS, 4), "bytes_per_cell": BYTES_TRANSPORT,
+                               "us_per_launch": round(tr_us, 2)},
         "solver_iteration_us": round(it_ms * 1e3, 2),
         "device_copy_gbs": round(copy_gbs, 1),
         "fp64_reference": {"J": J64, "its": args.ref_flat + args.ref_decay, "change_under_refinement": ref_change},
