@@ -56,6 +56,18 @@ __device__ __forceinline__ double lane_from_left(double x) {
 }
 
 // Sum over the 64 lanes of a wavefront (every lane gets the total).
+// Raw buffer resources with 32-bit byte offsets: an offset at or beyond the
+// resource size reads 0 and drops a store, in hardware, so out-of-grid and
+// masked lanes need no branch.  kOOB is above every resource size (the host
+// checks that each buffer is below 2 GiB).
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr uint32_t kOOB = 0x80000000u;
+constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, gfx9 data format
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
+}
+
 // MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin
 // (workgroup p runs on XCD p % 8), and each XCD has an L2 of its own.
 // xcd_remap turns the dispatch index p of nb workgroups into a logical index

@@ -29,6 +29,8 @@
 //  Adam: per-tensor clip_by_norm + legacy Adam on the fp32 master weights,
 //  then bf16 packing of the MFMA operands.
 #include "common.hpp"
+
+#include <cstdlib>
 #include "../../include/igm_amd.h"
 
 namespace igm {
@@ -151,17 +153,55 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
       for (int e = 0; e < 4; ++e) bias_r[cb][e] = bias[cb * 16 + 4 * (lane >> 4) + e];
   }
 
-  for (int64_t ti = t0; ti < t1; ++ti) {
-    const int tx = (int)(ti % ntx);
-    const int ty = (int)((ti / ntx) % nty);
-    const int b = (int)(ti / ((int64_t)nty * ntx));
-    const int x0 = tx * kTileW, y0 = ty * kTileH;
-    int ny, nx;
-    extent_of(ext, b, H, W, ny, nx);
-    const int64_t img = (int64_t)b * H * W;
-    const int y = y0 + wave;
+  // Staging of a tile's input rows y0-1 .. y0+4, pixels x0-1 .. x0+64 (zero
+  // outside the entry's extent): kChunks 16-B chunks, kPerThread per thread,
+  // all loads issued before any is used.  The next tile's loads are issued
+  // before the current tile's MFMAs, so they are in flight during them.
+  constexpr int kChunks = (kTileH + 2) * (kTileW + 2) * 4;
+  constexpr int kPerThread = (kChunks + 255) / 256;
+  uint4 stage[kPerThread];
+  struct Tile {
+    int b, y0, x0, ny, nx;
+    bool valid;
+  };
+  auto tile_of = [&](int64_t ti) {
+    Tile t;
+    t.x0 = (int)(ti % ntx) * kTileW;
+    t.y0 = (int)((ti / ntx) % nty) * kTileH;
+    t.b = (int)(ti / ((int64_t)nty * ntx));
+    extent_of(ext, t.b, H, W, t.ny, t.nx);
+    t.valid = t.y0 < t.ny && t.x0 < t.nx;
+    return t;
+  };
+  // branch-free staging loads of tile ti (all masked when ti >= t1 or the
+  // tile is padding): the destination registers stay fixed across the loop
+  const rsrc_t rin = make_rsrc(in, (uint32_t)((int64_t)B * H * W * kC * 2));
+  auto issue_loads = [&](const Tile& t, bool live) {
+    const int64_t img = (int64_t)t.b * H * W;
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+      const int c = tid + 256 * k;
+      const int chunk = c & 3;
+      const int px = (c >> 2) % (kTileW + 2);
+      const int row = (c >> 2) / (kTileW + 2);
+      const int yy = t.y0 - 1 + row, xx = t.x0 - 1 + px;
+      const bool ok = live && t.valid && c < kChunks && yy >= 0 && yy < t.ny && xx >= 0 && xx < t.nx;
+      const uint32_t off = ok ? (uint32_t)(((img + (int64_t)yy * W + xx) * kC + chunk * 8) * 2) : kOOB;
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(rin, off, 0, 0);
+      stage[k] = uint4{v[0], v[1], v[2], v[3]};
+    }
+  };
 
-    if (y0 >= ny || x0 >= nx) {  // tile outside the valid region: zeros
+  Tile cur = tile_of(t0);
+  issue_loads(cur, true);
+  for (int64_t ti = t0; ti < t1; ++ti) {
+    const int64_t img = (int64_t)cur.b * H * W;
+    const int y = cur.y0 + wave;
+    const int x0 = cur.x0;
+    const int ny = cur.ny, nx = cur.nx;
+    if (!cur.valid) {  // tile outside the valid region: zeros (block-uniform)
+      cur = tile_of(ti + 1 < t1 ? ti + 1 : ti);
+      issue_loads(cur, ti + 1 < t1);
       if (y < H) {
         const int x = x0 + lane;
         if (x < W) {
@@ -171,23 +211,24 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
           for (int q = 0; q < 4; ++q) o[q] = z;
         }
       }
-      continue;  // block-uniform
+      continue;
     }
-
+    // (the compute below is reached only through this path, after the wait
+    // for the staged loads, so the compiler places no memory waits in it)
     __syncthreads();  // the previous tile's fragment reads are done
-    // stage input rows y0-1 .. y0+4, pixels x0-1 .. x0+64 (zero outside)
-    constexpr int kChunks = (kTileH + 2) * (kTileW + 2) * 4;
-    for (int c = tid; c < kChunks; c += 256) {
-      const int chunk = c & 3;
-      const int px = (c >> 2) % (kTileW + 2);
-      const int row = (c >> 2) / (kTileW + 2);
-      const int yy = y0 - 1 + row, xx = x0 - 1 + px;
-      uint4 v = {0, 0, 0, 0};
-      if (yy >= 0 && yy < ny && xx >= 0 && xx < nx)
-        v = *reinterpret_cast<const uint4*>(in + (img + (int64_t)yy * W + xx) * kC + chunk * 8);
-      *reinterpret_cast<uint4*>(&tile[row][px][chunk * 8]) = v;
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+      const int c = tid + 256 * k;
+      if (c < kChunks) {
+        const int chunk = c & 3;
+        const int px = (c >> 2) % (kTileW + 2);
+        const int row = (c >> 2) / (kTileW + 2);
+        *reinterpret_cast<uint4*>(&tile[row][px][chunk * 8]) = stage[k];
+      }
     }
     __syncthreads();
+    cur = tile_of(ti + 1 < t1 ? ti + 1 : ti);
+    issue_loads(cur, ti + 1 < t1);
 
     f32x4 acc[4][2];
 #pragma unroll
@@ -197,13 +238,15 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
 
 #pragma unroll
     for (int pb = 0; pb < 4; ++pb) {
+      bf16x8 bfr[9];
+#pragma unroll
+      for (int t = 0; t < 9; ++t)
+        bfr[t] = *reinterpret_cast<const bf16x8*>(
+            &tile[wave + t / 3][pb * 16 + (lane & 15) + t % 3][8 * (lane >> 4)]);
 #pragma unroll
       for (int t = 0; t < 9; ++t) {
-        const int r = t / 3, s = t % 3;
-        const bf16x8 bfr =
-            *reinterpret_cast<const bf16x8*>(&tile[wave + r][pb * 16 + (lane & 15) + s][8 * (lane >> 4)]);
-        acc[pb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[t][0], bfr, acc[pb][0], 0, 0, 0);
-        acc[pb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[t][1], bfr, acc[pb][1], 0, 0, 0);
+        acc[pb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[t][0], bfr[t], acc[pb][0], 0, 0, 0);
+        acc[pb][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[t][1], bfr[t], acc[pb][1], 0, 0, 0);
       }
     }
 
@@ -239,10 +282,23 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
 
 // persistent grid for the 3x3 kernels: enough blocks to fill the chip a few
 // times over, each walking a contiguous run of tiles
+// 512 blocks (2 per CU, each pipelining a run of ~8 tiles): measured at
+// 1000 x 1000, 35.3 us per 32->32 layer against 36.6 / 39.5 / 46.2 us for
+// 1024 / 2048 / 4096 blocks and 48.6 us for 256.  (IGM_CONV_BLOCKS
+// overrides the target block count, for tuning sweeps.)
+inline int64_t conv_target_blocks() {
+  static int64_t n = [] {
+    const char* e = getenv("IGM_CONV_BLOCKS");
+    const long long r = e ? atoll(e) : 512;
+    return (int64_t)((r >= 64 && r <= (1 << 22)) ? r : 512);
+  }();
+  return n;
+}
+
 inline void conv_grid(int B, int H, int W, int* blocks, int* per) {
   const int64_t nty = (H + kTileH - 1) / kTileH, ntx = (W + kTileW - 1) / kTileW;
   const int64_t ntiles = (int64_t)B * nty * ntx;
-  constexpr int64_t kTarget = 2048;
+  const int64_t kTarget = conv_target_blocks();
   const int64_t p = (ntiles + kTarget - 1) / kTarget;
   *per = (int)(p < 1 ? 1 : p);
   *blocks = (int)((ntiles + *per - 1) / *per);
@@ -702,6 +758,7 @@ int igm_conv3x3_fwd_bf16(const void* in, const void* wpacked, const float* bias,
   IGM_CHECK_ARG(in && wpacked && bias && out, "null pointer");
   IGM_CHECK_ARG(in != out, "in and out must differ");
   IGM_CHECK_ARG(B >= 1 && H >= 1 && W >= 1 && B <= 65535, "bad sizes");
+  IGM_CHECK_ARG((int64_t)B * H * W * kC * 2 < (int64_t)kOOB, "activations above 2 GiB (32-bit buffer offsets)");
   int blocks, per;
   conv_grid(B, H, W, &blocks, &per);
   hipLaunchKernelGGL(conv3x3_mfma_kernel<0>, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
@@ -717,6 +774,7 @@ int igm_conv3x3_dgrad_bf16(const void* dz, const void* wpacked_dgrad, const void
   IGM_CHECK_ARG(dz && wpacked_dgrad && act_below && dz_below, "null pointer");
   IGM_CHECK_ARG(dz != dz_below, "in and out must differ");
   IGM_CHECK_ARG(B >= 1 && H >= 1 && W >= 1 && B <= 65535, "bad sizes");
+  IGM_CHECK_ARG((int64_t)B * H * W * kC * 2 < (int64_t)kOOB, "activations above 2 GiB (32-bit buffer offsets)");
   int blocks, per;
   conv_grid(B, H, W, &blocks, &per);
   hipLaunchKernelGGL(conv3x3_mfma_kernel<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream,

@@ -122,13 +122,6 @@ energy_coeffs_kernel(const T* __restrict__ thk, const T* __restrict__ usurf,
 // lane that does not own its node) the offset kOOB, so the zero boundary of
 // the stencil and the masked stores need no branches.
 
-using rsrc_t = __amdgpu_buffer_rsrc_t;
-constexpr uint32_t kOOB = 0x80000000u;      // > every resource size (host-checked < 2 GiB)
-constexpr int kRsrcWord3 = 0x00020000;      // raw buffer, gfx9 data format
-
-__device__ __forceinline__ rsrc_t make_rsrc(const void* p, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
-}
 // voff: per-lane byte offset (kOOB masks the lane); soff: wave-uniform byte
 // offset carried in an SGPR (the level offset), so it costs no VALU add.
 // Either way the range check sees an offset >= kOOB for a masked lane.

@@ -146,18 +146,39 @@ class Validator:
         return rel_err(U, V, self.Ur, self.Vr, self.mask)
 
 
+class WeightAverage:
+    """Exponential moving average of the flat fp32 weights (Polyak
+    averaging): ``swap`` exchanges it with the live weights (and repacks the
+    bf16 operands), so validation and saving can use the average."""
+
+    def __init__(self, cnn, decay):
+        self.cnn, self.decay = cnn, float(decay)
+        self.w = cnn.w.clone()
+
+    def update(self):
+        self.w.mul_(self.decay).add_(self.cnn.w, alpha=1.0 - self.decay)
+
+    def swap(self):
+        live = self.cnn.w.clone()
+        self.cnn.w.copy_(self.w)
+        self.w.copy_(live)
+        self.cnn.pack()
+
+
 def train(cnn, batcher, dzeta, phys, steps=None, seconds=None, lr=1e-4, lr_end=None, clipnorm=1.0,
-          epsilon=1e-7, log_every=100, val=None, on_best=None, log=print):
+          epsilon=1e-7, log_every=100, val=None, on_best=None, log=print, ema=None):
     """Energy-as-loss training for ``steps`` steps or ``seconds`` of wall
     time (whichever is given); returns the list of (step, mean COST per
     entry[, validation error]).
 
     lr decays geometrically from ``lr`` to ``lr_end`` over the run (constant
     if None).  ``val(cnn)`` (optional) is evaluated every log_every steps;
-    ``on_best(cnn, v)`` is called when it improves."""
+    ``on_best(cnn, v)`` is called when it improves.  With ``ema`` (a decay,
+    e.g. 0.999) the validation and ``on_best`` see the weight average."""
     assert (steps is None) != (seconds is None), "give steps or seconds"
     hist = []
     best = float("inf")
+    avg = WeightAverage(cnn, ema) if ema else None
     t0 = time.time()
     s = 0
     while True:
@@ -174,12 +195,16 @@ def train(cnn, batcher, dzeta, phys, steps=None, seconds=None, lr=1e-4, lr_end=N
             costs.append(e[:, 3])
         lr_s = lr if lr_end is None else lr * (lr_end / lr) ** min(1.0, frac)
         cnn.adam_step(lr_s, clipnorm=clipnorm, epsilon=epsilon)
+        if avg is not None:
+            avg.update()
         s += 1
         if s % log_every == 0 or s == 1:
             c = float(torch.cat(costs).mean())
             rec = [s, c]
             msg = f"step {s}: mean COST {c:.5f} lr {lr_s:.2e} ({time.time() - t0:.1f}s)"
             if val is not None:
+                if avg is not None:
+                    avg.swap()
                 v = float(val(cnn))
                 rec.append(v)
                 msg += f" | val {v:.4f}"
@@ -187,6 +212,8 @@ def train(cnn, batcher, dzeta, phys, steps=None, seconds=None, lr=1e-4, lr_end=N
                     best = v
                     if on_best is not None:
                         on_best(cnn, v)
+                if avg is not None:
+                    avg.swap()
             hist.append(tuple(rec))
             log(msg)
     return hist
@@ -221,6 +248,7 @@ def main():
     ap.add_argument("--crop-frac", type=float, default=0.5)
     ap.add_argument("--val-seed", type=int, default=None)
     ap.add_argument("--log-every", type=int, default=100)
+    ap.add_argument("--ema", type=float, default=None, help="validate/save an EMA of the weights (decay)")
     args = ap.parse_args()
     if args.steps is None and args.seconds is None:
         args.steps = 2000
@@ -254,7 +282,7 @@ def main():
         on_best = lambda m, v: save(m)
     hist = train(cnn, batcher, dzeta, ice.physics, steps=args.steps, seconds=args.seconds, lr=args.lr,
                  lr_end=args.lr_end, log_every=args.log_every, val=val, on_best=on_best,
-                 log=lambda m: print(m, flush=True))
+                 log=lambda m: print(m, flush=True), ema=args.ema)
     if val is None:
         save(cnn)
     else:

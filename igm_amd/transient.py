@@ -64,12 +64,18 @@ def run(cfg, state, log=None):
     return {k: np.asarray(v) for k, v in rec.items()}
 
 
-def transient_cfg(method="emulated", years=200.0, retrain_freq=10):
+def transient_cfg(method="emulated", years=200.0, retrain_freq=10, check_its=50, check_freq=10):
     """The transient workload's settings: CFL 0.3, step_max 1 yr, SMB =
-    min(0.005 (usurf - 2800), 2) m/yr, online retraining every 10 steps."""
+    min(0.005 (usurf - 2800), 2) m/yr, online retraining every 10 steps, and
+    the float64 energy-gap check every ``check_freq`` emulated updates
+    (``check_its`` solver iterations from the emulated field give the
+    comparison energy; a gap above 1 % falls back to a solve)."""
     cfg = config.default_cfg()
     cfg.processes.iceflow.method = method
     cfg.processes.iceflow.emulator.retrain_freq = retrain_freq
+    fb = cfg.processes.iceflow.emulator.fallback
+    fb.check_its = int(check_its)
+    fb.check_freq = int(check_freq)
     cfg.processes.time.start = 0.0
     cfg.processes.time.end = float(years)
     cfg.processes.time.save = float(years)
@@ -79,6 +85,11 @@ def transient_cfg(method="emulated", years=200.0, retrain_freq=10):
                                       [0.0, 0.005, 0.005, 2800.0, 2.0],
                                       [1.0e6, 0.005, 0.005, 2800.0, 2.0]]
     return cfg
+
+
+# years at which the two runs' ice volumes are compared (linear interpolation
+# between their own time steps)
+REPORT_YEARS = (1, 2, 5, 10, 15, 20, 30, 50, 100, 200)
 
 
 def main():
@@ -96,11 +107,14 @@ def main():
     ap.add_argument("--size", type=int, default=1500)
     ap.add_argument("--weights", default=None)
     ap.add_argument("--methods", nargs="*", default=["emulated", "solved"])
+    ap.add_argument("--check-its", type=int, default=50)
+    ap.add_argument("--check-freq", type=int, default=10)
     args = ap.parse_args()
     g = syn.domes1500() if args.size == 1500 else syn.domes1000()
     out = {"workload": f"domes{args.size}", "years": args.years}
     for method in args.methods:
-        cfg = transient_cfg(method=method, years=args.years)
+        cfg = transient_cfg(method=method, years=args.years, check_its=args.check_its,
+                            check_freq=args.check_freq)
         if args.weights:
             cfg.processes.iceflow.emulator.name = os.path.abspath(args.weights)
         st = make_state(g)
@@ -108,11 +122,15 @@ def main():
         rec = run(cfg, st, log=lambda s: print(f"[{method}] {s}", flush=True))
         wall = _time.perf_counter() - t0
         out[method] = {"steps": int(len(rec["t"])), "final_volume_km3": float(rec["volume"][-1]),
+                       "initial_volume_km3": float(rec["volume"][0]),
                        "iceflow_fields_per_s": float(1000.0 / np.mean(rec["iceflow_ms"])),
-                       "wall_s": wall, "fallbacks": int(rec["fallbacks"][-1])}
+                       "iceflow_ms_median": float(np.median(rec["iceflow_ms"])),
+                       "wall_s": wall, "fallbacks": int(rec["fallbacks"][-1]),
+                       "volume_at": {f"{t:g}": float(np.interp(t, rec["t"], rec["volume"]))
+                                     for t in REPORT_YEARS if t <= args.years}}
     if "emulated" in out and "solved" in out:
-        out["volume_rel_diff"] = abs(out["emulated"]["final_volume_km3"] - out["solved"]["final_volume_km3"]) / \
-            out["solved"]["final_volume_km3"]
+        ve, vs = out["emulated"]["volume_at"], out["solved"]["volume_at"]
+        out["volume_rel_diff_at"] = {t: (abs(ve[t] - vs[t]) / vs[t] if vs[t] > 0 else None) for t in vs}
     print(json.dumps(out), flush=True)
 
 
