@@ -306,8 +306,10 @@ energy_pass_kernel(const PassArgs<T> a) {
       const uint32_t e = (uint32_t)(row * a.Nx + ic) * (INTER ? 2u * NZ : 1u);
       return ok ? bbase + e * ES : kOOB;
     };
+    // node row `row` of U and V (rows this wave never uses -- past j1 --
+    // read nothing: the last step's prefetch would be wasted bytes)
     auto load_row = [&](T (&u)[NZ], T (&v)[NZ], int row) {
-      const uint32_t o = node_off(row, true);
+      const uint32_t o = node_off(row, row <= j1);
 #pragma unroll
       for (int k = 0; k < NZ; ++k) {
         u[k] = bload(rU, o, k * kstep, T());
@@ -326,7 +328,8 @@ energy_pass_kernel(const PassArgs<T> a) {
     // coefficient planes of cell row jc (kOOB rows read 0)
     const uint32_t ps = (uint32_t)a.plane * ES;
     auto load_coef = [&](T (&c)[kPlanes], int jc) {
-      const uint32_t co = (col_ok && jc >= 0 && jc < a.Ny) ? cbase + (uint32_t)(jc * a.Nx + ic) * ES : kOOB;
+      const uint32_t co =
+          (col_ok && jc >= 0 && jc < j1) ? cbase + (uint32_t)(jc * a.Nx + ic) * ES : kOOB;
 #pragma unroll
       for (int p = 0; p < kPlanes; ++p) c[p] = bload(rC, co, p * ps, T());
     };

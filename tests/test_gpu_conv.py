@@ -32,7 +32,9 @@ def nhwc(t):
     return t.permute(0, 2, 3, 1)
 
 
-@pytest.mark.parametrize("B,H,W", [(1, 7, 5), (2, 33, 70), (1, 64, 64), (3, 17, 130)])
+@pytest.mark.parametrize("B,H,W", [(1, 7, 5), (2, 33, 70), (1, 64, 64), (3, 17, 130),
+                                   # more tiles than persistent blocks: each block walks a run
+                                   (2, 255, 300)])
 def test_conv3x3_fwd_and_dgrad(dev, B, H, W):
     print(f"case conv3x3 B={B} H={H} W={W}", flush=True)
     g = torch.Generator().manual_seed(H * W)
@@ -221,6 +223,27 @@ def test_padded_batch_equals_single(dev):
         X[b, :ny, :nx] = xb
         singles.append(xb)
     X[0, 25:, :, :] = 7.0  # garbage outside the extents must not leak in
+    ext = torch.tensor(sizes, dtype=torch.int32, device=dev)
+    Y = cnn.forward(EM.nhwc_fields(X.to(dev).contiguous()), extents=ext).cpu()
+    for b, (ny, nx) in enumerate(sizes):
+        Yb = cnn.forward(EM.nhwc_fields(singles[b][None].to(dev).contiguous())).cpu()
+        assert torch.equal(Y[b, :ny, :nx], Yb[0])
+        assert float(Y[b, ny:].abs().sum()) == 0.0 and float(Y[b, :, nx:].abs().sum()) == 0.0
+
+
+def test_padded_batch_many_tiles(dev):
+    """Padded batch whose persistent blocks walk runs mixing valid and
+    padding-only tiles: every entry equals its own unpadded forward."""
+    cnn = EM.CNN(nz=10, nb_layers=3, device=dev, seed=9)
+    sizes = [(300, 200), (150, 260), (37, 61)]
+    H, W = 300, 260
+    X = torch.zeros((3, H, W, 5))
+    singles = []
+    for b, (ny, nx) in enumerate(sizes):
+        g = small_glacier(ny, nx, seed=30 + b)
+        xb = torch.stack([torch.as_tensor(g[k]) for k in ("thk", "usurf", "arrhenius", "slidingco", "dX")], -1).float()
+        X[b, :ny, :nx] = xb
+        singles.append(xb)
     ext = torch.tensor(sizes, dtype=torch.int32, device=dev)
     Y = cnn.forward(EM.nhwc_fields(X.to(dev).contiguous()), extents=ext).cpu()
     for b, (ny, nx) in enumerate(sizes):

@@ -15,7 +15,10 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("slope,ny,nx", [("superbee", 60, 90), ("minmod", 33, 17), ("godunov", 5, 300),
-                                         ("superbee", 1, 7)])
+                                         ("superbee", 1, 7),
+                                         # several 60-column strips and 8-row blocks, with remainders
+                                         ("superbee", 131, 245), ("minmod", 9, 61), ("godunov", 17, 121),
+                                         ("superbee", 64, 1), ("superbee", 8, 120)])
 def test_transport_matches_oracle(dev, slope, ny, nx):
     rng = np.random.default_rng(ny * nx)
     h = np.maximum(0, rng.normal(100, 120, (ny, nx))).astype(np.float32)
