@@ -111,6 +111,58 @@ conv_in_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
   }
 }
 
+// Gradient of the network with respect to its 5 input fields, through
+// layer 1: the transpose of conv_in_kernel's cross-correlation,
+//   dX[b, y, x, ci] = sum_{r, s, co} W[r][s][ci][co] * dz1[b, y - r + 1, x - s + 1, co],
+// zero padding, fp32 accumulation in a fixed order; dz1 (bf16 NHWC, 32
+// channels) is the gradient at layer 1's pre-activation output.  Zeros
+// outside an entry's extent.
+__global__ void __launch_bounds__(256)
+conv_in_dgrad_kernel(const bf16* __restrict__ dz1, const int32_t* __restrict__ ext, int H, int W,
+                     const float* __restrict__ w, float* __restrict__ dX) {
+  __shared__ float ws[9 * kCin * kC];
+  for (int i = threadIdx.x; i < 9 * kCin * kC; i += blockDim.x) ws[i] = w[i];
+  __syncthreads();
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  const int b = blockIdx.z;
+  if (x >= W) return;
+  int ny, nx;
+  extent_of(ext, b, H, W, ny, nx);
+  float* o = dX + (((int64_t)b * H + y) * W + x) * kCin;
+  float acc[kCin];
+#pragma unroll
+  for (int ci = 0; ci < kCin; ++ci) acc[ci] = 0.f;
+  if (y < ny && x < nx) {
+#pragma unroll 1
+    for (int r = 0; r < 3; ++r) {
+      const int yy = y - r + 1;
+#pragma unroll 1
+      for (int s = 0; s < 3; ++s) {
+        const int xx = x - s + 1;
+        if (yy < 0 || yy >= ny || xx < 0 || xx >= nx) continue;
+        const uint4* src = reinterpret_cast<const uint4*>(dz1 + (((int64_t)b * H + yy) * W + xx) * kC);
+        const float* wr = ws + (r * 3 + s) * kCin * kC;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint4 u = src[q];
+          const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int co = 8 * q + e;
+            const uint32_t bits = (e & 1) ? (wv[e >> 1] & 0xFFFF0000u) : (wv[e >> 1] << 16);
+            const float d = __uint_as_float(bits);
+#pragma unroll
+            for (int ci = 0; ci < kCin; ++ci) acc[ci] += d * wr[ci * kC + co];
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int ci = 0; ci < kCin; ++ci) o[ci] = acc[ci];
+}
+
 // ------------------------------------------------ 3x3 MFMA conv (fwd/dgrad) --
 // wt: [9][32 out][32 in] bf16.  MODE 0: out = LeakyReLU(conv + bias).
 // MODE 1 (dgrad): out = conv * (aux > 0 ? 1 : 0.01), aux = activation of the
@@ -860,6 +912,16 @@ int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const f
   IGM_CHECK_LAUNCH();
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, workspace + 9 * kCin * kC,
                      nblocks, n, kC, accumulate, db);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_conv_in_dgrad(const void* dz1, const float* w_hwio, const int32_t* extents, int32_t B,
+                      int32_t H, int32_t W, float* dX, void* stream) {
+  IGM_CHECK_ARG(dz1 && w_hwio && dX, "null pointer");
+  IGM_CHECK_ARG(B >= 1 && H >= 1 && W >= 1 && B <= 65535 && H <= 65535, "bad sizes");
+  hipLaunchKernelGGL(conv_in_dgrad_kernel, dim3((W + 255) / 256, H, B), dim3(256), 0,
+                     (hipStream_t)stream, (const bf16*)dz1, extents, H, W, w_hwio, dX);
   IGM_CHECK_LAUNCH();
   return 0;
 }

@@ -3,8 +3,8 @@
 // optionally, the legacy-Adam update, with a float64 energy reduction.
 //
 // Reference: igm/processes/iceflow/energy/cost_shear.py:103-179,
-// cost_gravity.py:204-249, cost_sliding_weertman.py:272-292, energy.py:12-22,
-// utils/gradient/compute_gradient.py:3-16, solve/solve.py:166-205.
+// cost_gravity.py:24-69, cost_sliding_weertman.py:23-43, energy.py:12-22,
+// utils/gradient/compute_gradient.py:3-16, solve/solve.py:45-84.
 //
 // Work decomposition ("column marching"): a wavefront owns a strip of 64
 // node columns and a block of consecutive node rows (the rows of a strip
@@ -213,7 +213,7 @@ struct PassArgs {
 // Sum of one batch entry's per-block partials in a fixed order (strided
 // per-thread sums, a wave butterfly, then the 4 waves in order) into
 // out[4] = {mean shear, mean sliding, mean gravity, COST}, then the
-// EarlyStopping rule (iceflow/utils.py:186-204) when es is given.  Run by one
+// EarlyStopping rule (iceflow/utils.py:92-111) when es is given.  Run by one
 // 256-thread block.  (It stays a kernel of its own: folding it into the pass
 // as "last block sums" needs a device-scope release per block, which on the
 // multi-XCD MI355X writes back the whole L2 -- measured 18 us per iteration

@@ -134,9 +134,11 @@ class CNN:
         return (Y, acts) if keep else Y
 
     # ----------------------------------------------------------- backward --
-    def backward(self, dY, acts, fields, extents=None, nblocks=512, accumulate=False):
+    def backward(self, dY, acts, fields, extents=None, nblocks=512, accumulate=False, input_grad=None):
         """Gradient of sum(dY * Y) w.r.t. all parameters, into self.g (added
-        to it when ``accumulate``)."""
+        to it when ``accumulate``).  ``input_grad`` (optional, fp32 (B, H, W,
+        5)) receives the gradient w.r.t. the input fields, the quantity data
+        assimilation differentiates through the emulator."""
         acc = 1 if accumulate else 0
         fptrs, (fb, fr, fc), B, H, W = fields
         ext = L.ptr(extents)
@@ -161,6 +163,10 @@ class CNN:
             dz, below = below, dz
         L.call("igm_conv_in_wgrad", *fptrs, fb, fr, fc, ext, B, H, W, L.ptr(dz), nblocks, L.ptr(ws), acc,
                L.ptr(self.kernel(0, self.g)), L.ptr(self.bias(0, self.g)), st)
+        if input_grad is not None:
+            assert input_grad.shape == (B, H, W, 5) and input_grad.dtype == torch.float32
+            assert input_grad.is_contiguous()
+            L.call("igm_conv_in_dgrad", L.ptr(dz), L.ptr(self.kernel(0)), ext, B, H, W, L.ptr(input_grad), st)
         return self.g
 
     # ---------------------------------------------------------- optimizer --

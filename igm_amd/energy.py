@@ -9,7 +9,7 @@ gradient (one fused sweep), so backward is a scale.
 
 Reference: igm/processes/iceflow/energy/energy.py:12-30 (iceflow_energy,
 iceflow_energy_XY), cost_shear.py, cost_gravity.py,
-cost_sliding_weertman.py; solve/solve.py:171-198.
+cost_sliding_weertman.py; solve/solve.py:50-77.
 """
 import ctypes as C
 

@@ -2,7 +2,7 @@
 mask + derived-velocity kernel.
 
 Reference: igm/processes/iceflow/utils.py:15-64 (initialize_iceflow_fields,
-get_velbase/get_velsurf/get_velbar), solve/solve.py:237-238 and
+get_velbase/get_velsurf/get_velbar), solve/solve.py:116-117 and
 emulate/emulate.py:143-152 (thk>0 mask).
 """
 import ctypes as C

@@ -52,7 +52,7 @@ def _check_supported(cfg):
     if ice.physics.force_negative_gravitational_energy:
         raise NotImplementedError("force_negative_gravitational_energy is not supported")
     if ice.solver.lbfgs:
-        # solve.py:292-293
+        # solve.py:171-172
         raise ValueError("solve_iceflow_lbfgs formely implemented, not working yet, will be updated.")
     if ice.emulator.exclude_borders != 0 or ice.emulator.network.multiple_window_size != 0:
         raise NotImplementedError("exclude_borders / multiple_window_size are not supported")
@@ -80,7 +80,7 @@ def initialize_iceflow_fields(cfg, state):
 
 
 def initialize_iceflow_solver(cfg, state):
-    """solve.py:132-141: one optimizer for all solves (its step counter persists)."""
+    """solve.py:11-20: one optimizer for all solves (its step counter persists)."""
     state.optimizer = slv.LegacyAdamState(lr=float(cfg.processes.iceflow.solver.step_size))
 
 
@@ -93,7 +93,7 @@ def energy_operator(cfg, state, dtype=torch.float32):
 
 
 def solve_iceflow(cfg, state, U, V, nbitmax=None):
-    """solve.py:143-240 on the GPU.  Returns (U, V) unmasked and the cost list."""
+    """solve.py:22-119 on the GPU.  Returns (U, V) unmasked and the cost list."""
     ice = cfg.processes.iceflow
     op = energy_operator(cfg, state)
     res = slv.solve(op, U, V, int(ice.solver.nbitmax if nbitmax is None else nbitmax), state.optimizer,
@@ -107,7 +107,7 @@ def _set_outputs(cfg, state, out):
 
 
 def update_iceflow_solved(cfg, state):
-    """solve.py:290-309."""
+    """solve.py:169-188."""
     ice = cfg.processes.iceflow
     res = solve_iceflow(cfg, state, state.U, state.V)
     out = velocity_outputs(res.U, res.V, state.thk, state.vert_weight, int(ice.numerics.Nz))

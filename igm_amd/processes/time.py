@@ -1,4 +1,4 @@
-"""Drop-in ``time`` process (igm/processes/time/time.py:11-69): CFL-adaptive
+"""Drop-in ``time`` process (igm/processes/time/time.py:11-65): CFL-adaptive
 time step from the GPU max-|velbar| reduction; scalar logic on the host."""
 import numpy as np
 import torch

@@ -1,11 +1,11 @@
 """Plain energy solver on the GPU: gradient descent with legacy Adam.
 
-Restates igm/processes/iceflow/solve/solve.py:132-240 MI355X-first:
+Restates igm/processes/iceflow/solve/solve.py:11-119 MI355X-first:
 
 * one fused kernel per iteration evaluates the energy, its adjoint and the
   Adam update (ping-pong velocity buffers, moments updated in place);
 * a tiny float64 reduction kernel per iteration produces COST and runs the
-  EarlyStopping rule (iceflow/utils.py:174-204) on the device;
+  EarlyStopping rule (iceflow/utils.py:80-111) on the device;
 * the host never waits on a COST value inside the loop: it reads the
   device stop flag every ``check_every`` iterations.  Iterations enqueued
   after the stop are no-ops (the kernels test the flag), so the result is
@@ -13,7 +13,7 @@ Restates igm/processes/iceflow/solve/solve.py:132-240 MI355X-first:
 
 Adam step constants follow TensorFlow's legacy Adam (see
 ``LegacyAdamState``); the step counter persists across solves like the
-optimizer created once by ``initialize_iceflow_solver`` (solve.py:132-141).
+optimizer created once by ``initialize_iceflow_solver`` (solve.py:11-20).
 """
 import ctypes as C
 

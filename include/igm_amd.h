@@ -42,7 +42,7 @@ typedef struct IgmPhysics {
   int32_t force_negative_gravitational_energy; /* must be 0 (not supported) */
 } IgmPhysics;
 
-/* Legacy-Keras Adam step constants (solve/solve.py:132-141, 205).
+/* Legacy-Keras Adam step constants (solve/solve.py:11-20, 84).
  * alpha = lr * sqrt(1 - beta2^t) / (1 - beta1^t), computed by the caller. */
 typedef struct IgmAdam {
   double alpha;
@@ -51,7 +51,7 @@ typedef struct IgmAdam {
   double epsilon;
 } IgmAdam;
 
-/* Device-side EarlyStopping state (iceflow/utils.py:174-204). */
+/* Device-side EarlyStopping state (iceflow/utils.py:80-111). */
 typedef struct IgmEarlyStop {
   double best_loss;
   double relative_min_delta;
@@ -75,8 +75,8 @@ int64_t igm_energy_workspace_doubles(int32_t nbatch, int32_t Ny, int32_t Nx);
  * stag4h of thk, B = 2 A^(-1/n) and slidingco, the staggered slopes of usurf
  * and of the bed (usurf - thk) from the dX field, the floating damping and
  * the 1/S mean weights.  Replaces the U-independent half of
- * energy/cost_shear.py:104-139, cost_gravity.py:205-216,
- * cost_sliding_weertman.py:273-285 and utils/gradient/compute_gradient.py:3-16.
+ * energy/cost_shear.py:104-139, cost_gravity.py:28-33,
+ * cost_sliding_weertman.py:27-36 and utils/gradient/compute_gradient.py:3-16.
  * Fields are read with element strides (fb, fr, fc); `extents` (device,
  * nullable) holds per-batch (ny, nx) valid sizes: cells outside get zero
  * coefficients, which masks them out of the energy.
@@ -94,10 +94,10 @@ int igm_energy_coeffs_f64(const double* thk, const double* usurf, const double* 
 
 /* One fused pass of the energy over (U, V).
  * mode 0: energy only (iceflow_energy + reduce_mean, energy/energy.py:12-22,
- *         solve/solve.py:178-180), per-block float64 partial sums.
+ *         solve/solve.py:57-59), per-block float64 partial sums.
  * mode 1: + gradient dJ/dU, dJ/dV into gU, gV (the GradientTape of
- *         solve/solve.py:198 and emulate/emulate.py:252).
- * mode 2: + legacy Adam update (solve/solve.py:205): reads U, V and the
+ *         solve/solve.py:77 and emulate/emulate.py:252).
+ * mode 2: + legacy Adam update (solve/solve.py:84): reads U, V and the
  *         moments; writes Uout, Vout (must not alias U, V) and the moments in
  *         place.  Planar velocity layout only.  moments: [nbatch][nz][Ny][Nx][4]
  *         = (m_U, v_U, m_V, v_V) per node and level, contiguous (the state of
@@ -121,17 +121,17 @@ int igm_energy_pass_f64(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, in
 
 /* Deterministic float64 reduction of the pass's partial sums:
  * out[b] = {mean shear, mean sliding, mean gravity, COST} per batch entry
- * (tf.reduce_mean + tf.add_n, solve/solve.py:178-180).
+ * (tf.reduce_mean + tf.add_n, solve/solve.py:57-59).
  * With `es` (device, nullable; nbatch must be 1): no-op once es->stopped;
  * otherwise stores COST in cost_hist[iter] (device, nullable) and applies
- * EarlyStopping.should_stop (solve/solve.py:233, iceflow/utils.py:186-204). */
+ * EarlyStopping.should_stop (solve/solve.py:112, iceflow/utils.py:80-111). */
 int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int32_t Nx,
                         double* out, IgmEarlyStop* es, double* cost_hist, int32_t iter,
                         void* stream);
 
 /* ---------------------------------------------------- velocity outputs -- */
 
-/* where(thk > 0, U, 0) and the derived 2-D fields (solve/solve.py:237-238,
+/* where(thk > 0, U, 0) and the derived 2-D fields (solve/solve.py:116-117,
  * 307-309; emulate/emulate.py:143-152; iceflow/utils.py:34-64):
  * Uo, Vo planar (nz, Ny, Nx) (nullable), uvelbase, vvelbase, uvelsurf,
  * vvelsurf, ubar, vbar (Ny, Nx) (each nullable).  Input layout by strides.
@@ -210,6 +210,13 @@ int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const f
                       int32_t B, int32_t H, int32_t W, const void* dz1, int32_t nblocks,
                       float* workspace, int32_t accumulate, float* dW_hwio, float* db,
                       void* stream);
+
+/* Gradient of the emulator with respect to its 5 input fields (the tape
+ * gradient through the network of data_assimilation/optimize/update.py:29-73),
+ * through layer 1: dX (fp32 NHWC [B][H][W][5]) from dz1, the bf16 gradient at
+ * layer 1's pre-activation output, and the layer-1 kernel w_hwio [3][3][5][32]. */
+int igm_conv_in_dgrad(const void* dz1, const float* w_hwio, const int32_t* extents, int32_t B,
+                      int32_t H, int32_t W, float* dX, void* stream);
 
 /* Legacy Keras Adam with per-tensor clipnorm on a flat fp32 parameter vector
  * (emulate.py:24-35, 267-269): tensor t spans [offs[t], offs[t+1]);

@@ -188,14 +188,14 @@ def iceflow_energy(U, V, fieldin, dzeta, phys=None):
 
 
 def total_cost(U, V, fieldin, dzeta, phys=None):
-    """solve/solve.py:171-180: COST = add_n([reduce_mean(e) for e in maps])."""
+    """solve/solve.py:50-59: COST = add_n([reduce_mean(e) for e in maps])."""
     maps = iceflow_energy(U, V, fieldin, dzeta, phys)
     means = [torch.mean(m) for m in maps]
     return means[0] + means[1] + means[2], means
 
 
 def energy_and_grad(U, V, fieldin, dzeta, phys=None):
-    """Energy and its gradient w.r.t. (U, V) -- the tape of solve.py:166-198.
+    """Energy and its gradient w.r.t. (U, V) -- the tape of solve.py:45-77.
 
     U, V: (Nz, Ny, Nx) tensors (no batch dim), any float dtype.
     Returns (COST, [shear, sliding, gravity] means, dU, dV).
@@ -208,7 +208,7 @@ def energy_and_grad(U, V, fieldin, dzeta, phys=None):
 
 
 def fieldin_tensors(thk, usurf, arrhenius, slidingco, dX, dtype=torch.float32):
-    """Build the (1, Ny, Nx) fieldin list the energy expects (solve.py:153)."""
+    """Build the (1, Ny, Nx) fieldin list the energy expects (solve.py:32)."""
     return [torch.as_tensor(f, dtype=dtype)[None] for f in (thk, usurf, arrhenius, slidingco, dX)]
 
 

@@ -2,7 +2,7 @@
 
 Restates ``igm/processes/iceflow/utils.py:34-64`` (get_vel*),
 ``utils.py:133-158`` (Y_to_UV / UV_to_Y / fieldin_to_X),
-``solve/solve.py:237-238`` and ``emulate/emulate.py:143-152`` (thk>0 mask),
+``solve/solve.py:116-117`` and ``emulate/emulate.py:143-152`` (thk>0 mask),
 ``diagnostic/diagnostic.py:74-84`` (computemisfit) and
 ``pretraining/pretraining.py:502-535`` (_computemisfitall).
 Pinned by construction tests (weights summing to one, masks, round trips).
@@ -12,7 +12,7 @@ import torch
 
 
 def mask_thk(U, thk):
-    """solve.py:237-238 / emulate.py:143-144: where(thk > 0, U, 0)."""
+    """solve.py:116-117 / emulate.py:143-144: where(thk > 0, U, 0)."""
     m = torch.as_tensor(thk) > 0
     return torch.where(m, U, torch.zeros_like(U))
 
@@ -34,7 +34,7 @@ def get_velbar(U, V, vert_weight):
 
 
 def velocity_outputs(U, V, thk, vert_weight):
-    """solve.py:237-238 + 307-309: mask then the six derived 2-D fields."""
+    """solve.py:116-117 + 186-188: mask then the six derived 2-D fields."""
     U = mask_thk(U, thk)
     V = mask_thk(V, thk)
     ub, vb = get_velbase(U, V)

@@ -1,17 +1,17 @@
 """Gradient-descent solver of the energy: legacy Keras Adam + EarlyStopping.
 
-Restates ``igm/processes/iceflow/solve/solve.py:143-240`` and
-``igm/processes/iceflow/utils.py:174-204`` (EarlyStopping) on the CPU.
+Restates ``igm/processes/iceflow/solve/solve.py:22-119`` and
+``igm/processes/iceflow/utils.py:80-111`` (EarlyStopping) on the CPU.
 
 Adam follows TensorFlow's legacy ``ResourceApplyAdam`` (what
-``tf.keras.optimizers.legacy.Adam`` calls, solve.py:139):
+``tf.keras.optimizers.legacy.Adam`` calls, solve.py:18-20):
   alpha = lr * sqrt(1 - beta2^t) / (1 - beta1^t)     (t = optimizer step + 1)
   m += (g - m) * (1 - beta1);  v += (g*g - v) * (1 - beta2)
   x -= (m * alpha) / (sqrt(v) + epsilon)
 with beta1=0.9, beta2=0.999, epsilon=1e-7 (Keras defaults).  The step
 counter is the optimizer's and persists across solves (the optimizer is
-created once in solve.py:132-141); moments are fresh per solve because the
-variables are new tf.Variables (solve.py:148-149).
+created once in solve.py:11-20); moments are fresh per solve because the
+variables are new tf.Variables (solve.py:27-28).
 
 Pins: Adam is checked against a hand-computed 2-step sequence and against
 torch.optim.Adam after the epsilon/bias-correction mapping; the solver is
@@ -57,7 +57,7 @@ class LegacyAdam:
 
 
 class EarlyStopping:
-    """iceflow/utils.py:174-204."""
+    """iceflow/utils.py:80-111."""
 
     def __init__(self, relative_min_delta=1e-3, patience=10):
         self.relative_min_delta = relative_min_delta
@@ -80,7 +80,7 @@ class EarlyStopping:
 
 def solve_iceflow(U, V, fieldin, dzeta, thk, nbitmax=100, lr=1.0, optimizer=None,
                   early_stopping=True, phys=None):
-    """solve.py:143-240: minimise the energy over (U, V) from the given start.
+    """solve.py:22-119: minimise the energy over (U, V) from the given start.
 
     U, V: (Nz, Ny, Nx) tensors (warm start); fieldin: list of (1, Ny, Nx).
     Returns (U, V masked by thk>0, list of COST values, iterations run).

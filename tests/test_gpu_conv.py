@@ -187,14 +187,16 @@ def test_network_backward_vs_autograd(dev):
     fields = EM.nhwc_fields(Xd)
     Y, acts = cnn.forward(fields, keep=True)
     dY = torch.randn(Y.shape, generator=torch.Generator().manual_seed(2)).float()
-    cnn.backward(dY.to(dev), acts, fields)
+    dX = torch.empty(tuple(Xd.shape), dtype=torch.float32, device=dev)
+    cnn.backward(dY.to(dev), acts, fields, input_grad=dX)
     grads = [(cnn.kernel(i, cnn.g).cpu(), cnn.bias(i, cnn.g).cpu()) for i in range(nl + 1)]
 
     def ste(x):  # bf16 rounding forward, identity backward
         return x + (bf(x.detach()).double() - x.detach())
 
     ps = []
-    h = nchw(X.double())
+    Xr = X.double().requires_grad_(True)
+    h = nchw(Xr)
     for li, (k, b) in enumerate(weights):
         kt = torch.as_tensor(k, dtype=torch.float64).requires_grad_(True)
         bt = torch.as_tensor(b, dtype=torch.float64).requires_grad_(True)
@@ -204,11 +206,15 @@ def test_network_backward_vs_autograd(dev):
         if li < nl:
             h = ste(F.leaky_relu(h, 0.01))
     loss = (nhwc(h) * dY.double()).sum()
-    gr = torch.autograd.grad(loss, ps)
+    gr = torch.autograd.grad(loss, ps + [Xr])
     for li in range(nl + 1):
         ek, eb = rel(grads[li][0], gr[2 * li]), rel(grads[li][1], gr[2 * li + 1])
         print(f"layer {li}: kernel grad rel {ek:.2e}, bias grad rel {eb:.2e}")
         assert ek <= 3e-2 and eb <= 3e-2
+    # gradient w.r.t. the input fields (data assimilation through the emulator)
+    ex = rel(dX.cpu(), gr[-1])
+    print(f"input grad rel {ex:.2e}")
+    assert ex <= 3e-2
 
 
 def test_padded_batch_equals_single(dev):
