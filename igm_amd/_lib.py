@@ -65,6 +65,8 @@ _SIGS = {
     "igm_conv_in_wgrad": (C.c_int, [VP, VP, VP, VP, VP, I64, I64, I64, VP, I32, I32, I32, VP, I32, VP, I32,
                                     VP, VP, VP]),
     "igm_conv_in_dgrad": (C.c_int, [VP, VP, VP, I32, I32, I32, VP, VP]),
+    "igm_vert_flow_kinematic_f32": (C.c_int, [I32, I32, I32, VP, VP, VP, VP, VP, VP, C.c_double, VP, VP, VP,
+                                              VP]),
     "igm_adam_clip_f32": (C.c_int, [VP, VP, VP, VP, I64, VP, I32, VP, F64, VP, VP]),
     "igm_pack_conv_weights": (C.c_int, [VP, VP, I32, VP, I32, VP, VP, VP, VP]),
     "igm_transport_f32": (C.c_int, [I32, I32, VP, VP, VP, VP, VP, F64, F64, F64, I32, F64, F64, VP, VP, VP,

@@ -235,6 +235,17 @@ int igm_pack_conv_weights(const float* w, const int64_t* offs3x3, int32_t nlayer
 
 /* ----------------------------------------------------------- transport -- */
 
+/* Vertical velocity from U, V: the default of vert_flow/vert_flow.py:21-56
+ * (Lagrange basis, version 2, kinematic; vert_flow_v2.py:17-37, 61-87,
+ * 125-147).  U, V planar (nz, ny, nx) float32; topg, thk (ny, nx); levels,
+ * vert_weight: host arrays of nz (vert_disc.py:10-13, 43-47); dx the grid
+ * spacing.  Writes W (nz, ny, nx) and, when given, W[0] and W[nz-1] into
+ * wvelbase, wvelsurf (vert_flow.py:55-56).  ny, nx >= 2. */
+int igm_vert_flow_kinematic_f32(int32_t nz, int32_t ny, int32_t nx, const float* U, const float* V,
+                                const float* topg, const float* thk, const double* levels,
+                                const double* vert_weight, double dx, float* W, float* wvelbase,
+                                float* wvelsurf, void* stream);
+
 /* Fused thickness update of the transport step (thk.py:27-54 with
  * compute_divflux_slope_limiter.py:9-65): slope-limited upwind flux
  * divergence of (ubar, vbar) * thk, thk_out = max(thk + dt (smb - div), 0),
