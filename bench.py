@@ -47,7 +47,8 @@ BYTES_TRANSPORT = 28.0
 # `bench.py --kernels-only`, scaled by a 1 GiB copy (the roofline's `traffic`)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_domes1000.json")
 # schedule family searched for both paths: initial step, then total iterations
-LR0S = (3.0, 10.0, 30.0)
+LR0S = (30.0, 100.0, 300.0, 1000.0)
+FLATS = (0.1, 0.3)  # fraction of the iterations at the initial step
 CPU_SAMPLE_S = 15.0  # wall time of the CPU solver-iteration sample
 REFINE_TOTALS = (100, 200, 300, 500, 700, 1000, 1500, 2000, 3000, 4500, 7000, 10000, 14000)
 SOLVER_TOTALS = (1000, 1500, 2000, 3000, 4500, 7000, 10000, 14000, 20000)
@@ -244,23 +245,24 @@ def main():
     z = torch.zeros((nz, Ny, Nx), dtype=torch.float32, device=dev)
 
     def schedule_search(U0, V0, totals):
-        """Shortest schedule of the family {lr0 in LR0S; 30 % flat, then a
-        geometric decay to 1e-3} from (U0, V0) that ends <= 1 %: the first
-        total of ``totals`` that does, then bisected (3 steps, to 50
-        iterations) against the total before it.  Returns (its, lr0, err)
-        (the last tried if none reaches 1 %)."""
+        """Shortest schedule of the family {lr0 in LR0S; a fraction in FLATS
+        at lr0, then a geometric decay to 1e-3} from (U0, V0) that ends
+        <= 1 %: the first total of ``totals`` that does, then bisected (3
+        steps, to 50 iterations) against the total before it.  Returns
+        (its, lr0, err, flat fraction) (the last tried if none reaches 1 %)."""
 
         def trial(total):
             if total == 0:
-                return (0, None, rel_err(U0, V0, Ur, Vr, mask))
+                return (0, None, rel_err(U0, V0, Ur, Vr, mask), None)
             cand = []
             for lr0 in LR0S:
-                n_flat = int(0.3 * total)
-                r = solve(op32, U0, V0, total, LegacyAdamState(lr=lr0), early_stopping=False,
-                          lr_schedule=adam_schedule(lr0, n_flat, total - n_flat))
-                cand.append((total, lr0, rel_err(r.U, r.V, Ur, Vr, mask)))
+                for ff in FLATS:
+                    n_flat = int(ff * total)
+                    r = solve(op32, U0, V0, total, LegacyAdamState(lr=lr0), early_stopping=False,
+                              lr_schedule=adam_schedule(lr0, n_flat, total - n_flat))
+                    cand.append((total, lr0, rel_err(r.U, r.V, Ur, Vr, mask), ff))
             best = min(cand, key=lambda c: c[2])
-            log(f"[bench]   {total} its: best rel err {best[2]:.4f} (lr0 {best[1]})")
+            log(f"[bench]   {total} its: best rel err {best[2]:.4f} (lr0 {best[1]}, flat {best[3]})")
             return best
 
         lo, best = None, None
@@ -297,10 +299,11 @@ def main():
             EM.update_iceflow_emulated(cfg, state)
     inf_ms = tm.ms / args.steps
     log(f"[bench] emulator inference {inf_ms:.3f} ms, rel err {inf_err:.4f}; searching the polish")
-    n_ref, lr_ref, _ = schedule_search(U_emu, V_emu, (0,) + REFINE_TOTALS)
+    n_ref, lr_ref, _, ff_ref = schedule_search(U_emu, V_emu, (0,) + REFINE_TOTALS)
     ice.emulator.refine.its = int(n_ref)
     if lr_ref is not None:
         ice.emulator.refine.lr = float(lr_ref)
+        ice.emulator.refine.flat_frac = float(ff_ref)
 
     def emulator_step():
         EM.update_iceflow_emulated(cfg, state)
@@ -337,9 +340,9 @@ def main():
     ref_settings = {"lr": 1.0, "error_curve": curve}
     # (b) the same method and schedule family as the polish, from zero
     log("[bench] searching the solver schedule from zero")
-    n1, lr1, solver_err = schedule_search(z, z, SOLVER_TOTALS)
+    n1, lr1, solver_err, ff1 = schedule_search(z, z, SOLVER_TOTALS)
     solver_reached = solver_err <= 0.01
-    n_flat1 = int(0.3 * n1)
+    n_flat1 = int(ff1 * n1)
     sched = adam_schedule(lr1, n_flat1, n1 - n_flat1)
     solve(op32, z, z.clone(), 20, LegacyAdamState(lr=1.0), early_stopping=False)
     with Timer() as tm:
@@ -440,14 +443,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16",
+        "dtype": "bf16+fp32",  # bf16 MFMA convolutions, fp32 energy/adjoint/Adam polish
         "data": "synthetic",
         "config": {"workload": "domes1000: 1000x1000 grid, dx=50 m, Nz=10, 20 seeded domes on a random-field bed; "
                                "emulated iceflow update (bf16 MFMA CNN 16x3x3x32 + 1x1, fp32 first layer) "
                                "polished by the shortest warm-started fp32 energy-solver schedule that reaches "
                                "<= 1 % rel err vs the float64 minimiser, plus the float64 energy-gap check",
                    "weights": os.path.basename(wpath),
-                   "polish_its": int(n_ref), "polish_lr0": lr_ref,
+                   "polish_its": int(n_ref), "polish_lr0": lr_ref, "polish_flat_frac": ff_ref,
                    "fallback_check_its": int(ice.emulator.fallback.check_its)},
         "roofline": roofline,
         "cpu_baseline": cpu,
