@@ -191,6 +191,18 @@ struct PassArgs {
   const T* hscale;
   double* partials;
   const int32_t* stop;
+  // finalize of the previous iteration, done by every block at its start
+  // (prev_partials != nullptr; nbatch 1): the sums of prev_partials give the
+  // previous COST, the EarlyStopping rule is applied to a copy of *es_in, and
+  // the block decides from it whether this iteration runs; block 0 writes the
+  // new state to *es_out (never *es_in, which other blocks are reading), the
+  // energies to fin_out and the COST to cost_hist[prev_iter]
+  const double* prev_partials;
+  const IgmEarlyStop* es_in;
+  IgmEarlyStop* es_out;
+  double* fin_out;
+  double* cost_hist;
+  int prev_iter;
   uint32_t vel_bytes;   // size of every velocity-shaped resource
   uint32_t mom_bytes;
   uint32_t coef_bytes;
@@ -214,12 +226,33 @@ struct PassArgs {
 // per-thread sums, a wave butterfly, then the 4 waves in order) into
 // out[4] = {mean shear, mean sliding, mean gravity, COST}, then the
 // EarlyStopping rule (iceflow/utils.py:92-111) when es is given.  Run by one
-// 256-thread block.  (It stays a kernel of its own: folding it into the pass
-// as "last block sums" needs a device-scope release per block, which on the
-// multi-XCD MI355X writes back the whole L2 -- measured 18 us per iteration
-// against 7 us for this launch.)
-__device__ void finalize_entry(const double* p, int nblk, double* out, IgmEarlyStop* es,
-                               double* cost_hist, int iter) {
+// 256-thread block.  The solver folds it into the next iteration's pass
+// instead (every block redoes the sum of the previous pass's partials, which
+// the kernel boundary has made visible): a "last block sums" fold inside the
+// same pass would need a device-scope release per block, which on the
+// multi-XCD MI355X writes back the whole L2 (measured 18 us per iteration).
+__device__ __forceinline__ void es_apply(IgmEarlyStop& s, double cost, int iter) {
+  s.iters_done = iter + 1;
+  if (!s.has_best) {
+    s.best_loss = cost;
+    s.has_best = 1;
+  } else {
+    const double rel = (s.best_loss - cost) / fabs(s.best_loss);
+    if (rel > s.relative_min_delta) {
+      s.best_loss = cost;
+      s.wait = 0;
+    } else {
+      s.wait += 1;
+      if (s.wait >= s.patience) {
+        s.stopped = 1;
+        s.stop_iter = iter;
+      }
+    }
+  }
+}
+
+// (shear, sliding, gravity) sums of one entry's partials, in every thread
+__device__ void block_sum_partials(const double* p, int nblk, double (&r)[3]) {
   __shared__ double fin[4][3];
   double s0 = 0, s1 = 0, s2 = 0;
   for (int i = threadIdx.x; i < nblk; i += 256) {
@@ -236,34 +269,25 @@ __device__ void finalize_entry(const double* p, int nblk, double* out, IgmEarlyS
     fin[threadIdx.x >> 6][2] = s2;
   }
   __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r[c] = ((fin[0][c] + fin[1][c]) + fin[2][c]) + fin[3][c];
+}
+
+__device__ void finalize_entry(const double* p, int nblk, double* out, IgmEarlyStop* es,
+                               double* cost_hist, int iter) {
+  double r[3];
+  block_sum_partials(p, nblk, r);
   if (threadIdx.x == 0) {
-    const double sh = ((fin[0][0] + fin[1][0]) + fin[2][0]) + fin[3][0];
-    const double sl = ((fin[0][1] + fin[1][1]) + fin[2][1]) + fin[3][1];
-    const double gr = ((fin[0][2] + fin[1][2]) + fin[2][2]) + fin[3][2];
-    const double cost = (sh + sl) + gr;  // tf.add_n in energy_components order
-    out[0] = sh;
-    out[1] = sl;
-    out[2] = gr;
+    const double cost = (r[0] + r[1]) + r[2];  // tf.add_n in energy_components order
+    out[0] = r[0];
+    out[1] = r[1];
+    out[2] = r[2];
     out[3] = cost;
     if (es != nullptr) {
       if (cost_hist != nullptr) cost_hist[iter] = cost;
-      es->iters_done = iter + 1;
-      if (!es->has_best) {
-        es->best_loss = cost;
-        es->has_best = 1;
-      } else {
-        const double rel = (es->best_loss - cost) / fabs(es->best_loss);
-        if (rel > es->relative_min_delta) {
-          es->best_loss = cost;
-          es->wait = 0;
-        } else {
-          es->wait += 1;
-          if (es->wait >= es->patience) {
-            es->stopped = 1;
-            es->stop_iter = iter;
-          }
-        }
-      }
+      IgmEarlyStop st = *es;
+      es_apply(st, cost, iter);
+      *es = st;
     }
   }
 }
@@ -279,7 +303,26 @@ energy_pass_kernel(const PassArgs<T> a) {
   const int wid = lblk * kWavesPerBlock + (threadIdx.x >> 6);
   const int b = blockIdx.y;
   double e_sh = 0.0, e_sl = 0.0, e_gr = 0.0;
-  const bool stopped = (a.stop != nullptr) && (*a.stop != 0);
+  bool stopped = (a.stop != nullptr) && (*a.stop != 0);
+  if (a.prev_partials != nullptr) {
+    double r[3];
+    block_sum_partials(a.prev_partials, a.blocks_per_batch, r);
+    const double cost = (r[0] + r[1]) + r[2];
+    IgmEarlyStop st = *a.es_in;
+    const bool was = st.stopped != 0;
+    if (!was) es_apply(st, cost, a.prev_iter);
+    if (lblk == 0 && threadIdx.x == 0) {
+      *a.es_out = st;  // (when already stopped: carried over unchanged)
+      if (!was) {
+        a.fin_out[0] = r[0];
+        a.fin_out[1] = r[1];
+        a.fin_out[2] = r[2];
+        a.fin_out[3] = cost;
+        if (a.cost_hist != nullptr) a.cost_hist[a.prev_iter] = cost;
+      }
+    }
+    stopped = stopped || (st.stopped != 0);
+  }
   constexpr uint32_t ES = sizeof(T);
 
   if (!stopped && wid < a.nstrips * a.nrowblk) {
@@ -646,7 +689,9 @@ template <typename T>
 int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T* V, int64_t bs,
               int64_t ks, int64_t rs, int64_t cs, const T* coef, const T* hscale,
               const double* dzeta, const IgmPhysics* phys, T* gU, T* gV, T* Uo, T* Vo, T* mom,
-              const IgmAdam* adam, double* partials, const int32_t* stop, void* stream) {
+              const IgmAdam* adam, double* partials, const int32_t* stop,
+              const double* prev_partials, const IgmEarlyStop* es_in, IgmEarlyStop* es_out,
+              double* fin_out, double* cost_hist, int prev_iter, void* stream) {
   IGM_CHECK_ARG(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
   IGM_CHECK_ARG(U && V && coef && hscale && dzeta && phys && partials, "null pointer");
   IGM_CHECK_ARG(nbatch >= 1 && Ny >= 2 && Nx >= 2, "need nbatch >= 1 and a grid of at least 2 x 2");
@@ -676,6 +721,13 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
   a.U = U; a.V = V; a.gU = gU; a.gV = gV; a.Uo = Uo; a.Vo = Vo;
   a.mom = mom;
   a.coef = coef; a.hscale = hscale; a.partials = partials; a.stop = stop;
+  if (prev_partials) {
+    IGM_CHECK_ARG(nbatch == 1, "the fused finalize needs nbatch == 1");
+    IGM_CHECK_ARG(es_in && es_out && fin_out && es_in != es_out, "fused finalize needs es_in != es_out and out");
+    IGM_CHECK_ARG(prev_partials != partials, "prev_partials must not alias partials");
+  }
+  a.prev_partials = prev_partials; a.es_in = es_in; a.es_out = es_out; a.fin_out = fin_out;
+  a.cost_hist = cost_hist; a.prev_iter = prev_iter;
   a.vel_bytes = (uint32_t)vel_bytes;
   a.coef_bytes = (uint32_t)coef_bytes;
   a.mom_bytes = (uint32_t)mom_bytes;
@@ -826,10 +878,14 @@ int igm_energy_pass_f32(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, in
                         int64_t rstride, int64_t cstride, const float* coef, const float* hscale,
                         const double* dzeta, const IgmPhysics* phys, float* gU, float* gV,
                         float* Uout, float* Vout, float* moments, const IgmAdam* adam,
-                        double* partials, const int32_t* stop_flag, void* stream) {
+                        double* partials, const int32_t* stop_flag,
+                        const double* prev_partials, const IgmEarlyStop* es_in,
+                        IgmEarlyStop* es_out, double* out, double* cost_hist, int32_t prev_iter,
+                        void* stream) {
   return igm::pass_impl<float>(mode, nz, nbatch, Ny, Nx, U, V, bstride, kstride, rstride, cstride,
                                coef, hscale, dzeta, phys, gU, gV, Uout, Vout, moments, adam,
-                               partials, stop_flag, stream);
+                               partials, stop_flag, prev_partials, es_in, es_out, out, cost_hist,
+                               prev_iter, stream);
 }
 int igm_energy_pass_f64(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
                         const double* U, const double* V, int64_t bstride, int64_t kstride,
@@ -837,10 +893,13 @@ int igm_energy_pass_f64(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, in
                         const double* hscale, const double* dzeta, const IgmPhysics* phys,
                         double* gU, double* gV, double* Uout, double* Vout, double* moments,
                         const IgmAdam* adam, double* partials, const int32_t* stop_flag,
+                        const double* prev_partials, const IgmEarlyStop* es_in,
+                        IgmEarlyStop* es_out, double* out, double* cost_hist, int32_t prev_iter,
                         void* stream) {
   return igm::pass_impl<double>(mode, nz, nbatch, Ny, Nx, U, V, bstride, kstride, rstride,
                                 cstride, coef, hscale, dzeta, phys, gU, gV, Uout, Vout, moments,
-                                adam, partials, stop_flag, stream);
+                                adam, partials, stop_flag, prev_partials, es_in, es_out, out,
+                                cost_hist, prev_iter, stream);
 }
 
 int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int32_t Nx,

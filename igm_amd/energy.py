@@ -100,9 +100,13 @@ class EnergyOperator:
 
     # -- raw passes ---------------------------------------------------------
     def run_pass(self, mode, U, V=None, gU=None, gV=None, Uo=None, Vo=None, moments=None, adam=None,
-                 stop_flag=None):
+                 stop_flag=None, partials=None, prev=None):
         """One pass (mode 0 energy, 1 + gradient, 2 + Adam step).  ``moments``
-        (mode 2): (B, Nz, Ny, Nx, 4) Adam slots (m_U, v_U, m_V, v_V)."""
+        (mode 2): (B, Nz, Ny, Nx, 4) Adam slots (m_U, v_U, m_V, v_V).
+        ``partials``: the per-block sums buffer (default: the operator's own).
+        ``prev`` (optional): (prev_partials, es_in, es_out, cost_hist,
+        prev_iter) -- the previous iteration's finalize, done by the pass
+        itself (see ``solver.solve``)."""
         up, vp, bs, ks, rs, cs = velocity_strides(U, V, self.nz)
         if gU is not None and gV is None:  # gradient in Y layout
             gu, gv = velocity_strides(gU, None, self.nz)[:2]
@@ -112,11 +116,15 @@ class EnergyOperator:
                bs, ks, rs, cs, L.ptr(self.coef), L.ptr(self.hscale), C.addressof(self.dzeta),
                C.addressof(self.phys), gu, gv, L.ptr(Uo), L.ptr(Vo), L.ptr(moments),
                C.addressof(adam) if adam is not None else None,
-               L.ptr(self.partials), stop_flag, L.stream_ptr())
+               L.ptr(self.partials if partials is None else partials), stop_flag,
+               *((L.ptr(prev[0]), prev[1], prev[2], L.ptr(self.out), L.ptr(prev[3]), prev[4])
+                 if prev is not None else (None, None, None, None, None, 0)),
+               L.stream_ptr())
 
-    def finalize(self, out=None, es=None, cost_hist=None, it=0):
+    def finalize(self, out=None, es=None, cost_hist=None, it=0, partials=None):
         out = self.out if out is None else out
-        L.call("igm_energy_finalize", L.ptr(self.partials), self.B, self.Ny, self.Nx, L.ptr(out),
+        p = self.partials if partials is None else partials
+        L.call("igm_energy_finalize", L.ptr(p), self.B, self.Ny, self.Nx, L.ptr(out),
                es, L.ptr(cost_hist), it, L.stream_ptr())
         return out
 

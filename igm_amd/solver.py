@@ -4,8 +4,9 @@ Restates igm/processes/iceflow/solve/solve.py:11-119 MI355X-first:
 
 * one fused kernel per iteration evaluates the energy, its adjoint and the
   Adam update (ping-pong velocity buffers, moments updated in place);
-* a tiny float64 reduction kernel per iteration produces COST and runs the
-  EarlyStopping rule (iceflow/utils.py:80-111) on the device;
+* each pass first reduces the previous iteration's float64 COST and runs
+  the EarlyStopping rule (iceflow/utils.py:80-111) on the device, every
+  block for itself, so an iteration is one kernel launch;
 * the host never waits on a COST value inside the loop: it reads the
   device stop flag every ``check_every`` iterations.  Iterations enqueued
   after the stop are no-ops (the kernels test the flag), so the result is
@@ -66,28 +67,43 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     es_host = L.IgmEarlyStop(best_loss=0.0, relative_min_delta=relative_min_delta, has_best=0,
                              wait=0, patience=patience if early_stopping else 2**30, stopped=0,
                              stop_iter=-1, iters_done=0)
-    es_dev = torch.frombuffer(bytearray(bytes(es_host)), dtype=torch.uint8).to(dev)
-    es_ptr = es_dev.data_ptr()
-    stop_ptr = es_ptr + L.EARLYSTOP_STOPPED_OFFSET
+    # Two EarlyStopping states and two partial-sum buffers: pass i finalizes
+    # iteration i-1 itself (every block sums its partials and applies the
+    # rule), reading state E[i % 2] and writing E[(i - 1) % 2], so a state is
+    # never written while blocks of the same pass read it; the pass writes its
+    # own partials to P[i % 2].  The state of the last launched pass is
+    # E[n % 2], where a separate finalize then accounts for iteration n - 1.
+    es_size = C.sizeof(L.IgmEarlyStop)
+    es_dev = torch.frombuffer(bytearray(bytes(es_host) * 2), dtype=torch.uint8).to(dev)
+    E = [es_dev.data_ptr(), es_dev.data_ptr() + es_size]
+    P = [op.partials, torch.zeros_like(op.partials)]
     hist = torch.zeros(max(1, nbitmax), dtype=torch.float64, device=dev)
     adam = L.IgmAdam(alpha=0.0, beta1=optimizer.beta1, beta2=optimizer.beta2,
                      epsilon=optimizer.epsilon)
     t0 = optimizer.iterations
     lr0 = optimizer.lr
+    n_launched = 0
     for i in range(nbitmax):
         if lr_schedule is not None:
             optimizer.lr = lr_schedule(i)
         adam.alpha = optimizer.alpha(t0 + i + 1, dt)
         src, dst = i % 2, (i + 1) % 2
+        prev = None if i == 0 else (P[(i - 1) % 2], E[i % 2], E[(i - 1) % 2], hist, i - 1)
         op.run_pass(2, Ub[src], Vb[src], Uo=Ub[dst], Vo=Vb[dst], moments=mom, adam=adam,
-                    stop_flag=stop_ptr)
-        op.finalize(es=es_ptr, cost_hist=hist, it=i)
-        if early_stopping and (i + 1) % check_every == 0:
-            if int(es_dev[L.EARLYSTOP_STOPPED_OFFSET:L.EARLYSTOP_STOPPED_OFFSET + 4]
-                   .view(torch.int32).item()) != 0:
+                    partials=P[i % 2], prev=prev)
+        n_launched = i + 1
+        if early_stopping and i > 0 and (i + 1) % check_every == 0:
+            o = ((i - 1) % 2) * es_size + L.EARLYSTOP_STOPPED_OFFSET
+            if int(es_dev[o:o + 4].view(torch.int32).item()) != 0:
                 break
+    if n_launched > 0:
+        last = n_launched % 2
+        op.finalize(es=E[last], cost_hist=hist, it=n_launched - 1, partials=P[(n_launched - 1) % 2])
+    else:
+        last = 0
     optimizer.lr = lr0
-    st = L.IgmEarlyStop.from_buffer_copy(bytes(es_dev.cpu().numpy().tobytes()))
+    raw = bytes(es_dev.cpu().numpy().tobytes())
+    st = L.IgmEarlyStop.from_buffer_copy(raw[last * es_size:(last + 1) * es_size])
     n = int(st.iters_done)
     optimizer.iterations = t0 + n
     res = n % 2

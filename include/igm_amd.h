@@ -104,19 +104,31 @@ int igm_energy_coeffs_f64(const double* thk, const double* usurf, const double* 
  *         the reference's Adam slots, one 16-B record per node-level).
  * dzeta: host array of nz-1 layer thicknesses (vert_disc.py:26-35).
  * stop_flag (device, nullable): when *stop_flag != 0 the pass does nothing.
- * partials: igm_energy_workspace_doubles() doubles (per-block sums). */
+ * partials: igm_energy_workspace_doubles() doubles (per-block sums).
+ * prev_partials (device, nullable; nbatch 1): the partials of the previous
+ *         iteration's pass (another buffer).  Every block first does what
+ *         igm_energy_finalize does for that iteration: COST (solve.py:57-59)
+ *         and the EarlyStopping rule (solve.py:112) applied to a copy of
+ *         *es_in, and skips this iteration if it says stop; block 0 writes
+ *         the new state to *es_out (!= es_in), the energies to out[4] and
+ *         the COST to cost_hist[prev_iter].  One launch per solver
+ *         iteration; the last iteration's finalize stays a separate call. */
 int igm_energy_pass_f32(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
                         const float* U, const float* V, int64_t bstride, int64_t kstride,
                         int64_t rstride, int64_t cstride, const float* coef, const float* hscale,
                         const double* dzeta, const IgmPhysics* phys, float* gU, float* gV,
                         float* Uout, float* Vout, float* moments, const IgmAdam* adam,
-                        double* partials, const int32_t* stop_flag, void* stream);
+                        double* partials, const int32_t* stop_flag, const double* prev_partials,
+                        const IgmEarlyStop* es_in, IgmEarlyStop* es_out, double* out,
+                        double* cost_hist, int32_t prev_iter, void* stream);
 int igm_energy_pass_f64(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
                         const double* U, const double* V, int64_t bstride, int64_t kstride,
                         int64_t rstride, int64_t cstride, const double* coef,
                         const double* hscale, const double* dzeta, const IgmPhysics* phys,
                         double* gU, double* gV, double* Uout, double* Vout, double* moments,
                         const IgmAdam* adam, double* partials, const int32_t* stop_flag,
+                        const double* prev_partials, const IgmEarlyStop* es_in,
+                        IgmEarlyStop* es_out, double* out, double* cost_hist, int32_t prev_iter,
                         void* stream);
 
 /* Deterministic float64 reduction of the pass's partial sums:
