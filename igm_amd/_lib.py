@@ -51,6 +51,9 @@ _SIGS = {
     "igm_energy_pass_f64": (C.c_int, [I32, I32, I32, I32, I32, VP, VP, I64, I64, I64, I64, VP, VP, VP, VP,
                                       VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, I32, VP]),
     "igm_energy_finalize": (C.c_int, [VP, I32, I32, I32, VP, VP, VP, I32, VP]),
+    "igm_energy_pass2_f32": (C.c_int, [I32, I32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
+                                       VP, VP, I32, VP, VP, VP, VP, I32, VP]),
+    "igm_energy_finalize_pair": (C.c_int, [VP, I32, I32, VP, VP, VP, I32, VP]),
     "igm_velocity_outputs_f32": (C.c_int, [I32, I32, I32, VP, VP, I64, I64, I64, VP, VP, VP, VP, VP, VP, VP,
                                            VP, VP, VP, VP]),
     "igm_velocity_outputs_f64": (C.c_int, [I32, I32, I32, VP, VP, I64, I64, I64, VP, VP, VP, VP, VP, VP, VP,

@@ -22,10 +22,8 @@
 //
 // Per-cell coefficients that do not depend on (U, V) are computed once per
 // solve by energy_coeffs_kernel (7 planes, 28 B per cell).
-#include "common.hpp"
-#include "../../include/igm_amd.h"
+#include "energy_pass.hpp"
 
-#include <cmath>
 #include <mutex>
 
 namespace igm {
@@ -35,11 +33,6 @@ thread_local std::string g_last_error;
 }
 void set_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }
-
-constexpr int kPlanes = 7;     // q, w, gx, gy, cs, bx, by
-constexpr int kStripOut = 62;  // owned columns per wave
-constexpr int kPassBlock = 256;
-constexpr int kWavesPerBlock = kPassBlock / kWave;
 
 // --------------------------------------------------------- coefficients --
 //
@@ -114,182 +107,37 @@ energy_coeffs_kernel(const T* __restrict__ thk, const T* __restrict__ usurf,
   cb[6 * plane] = by;
 }
 
-// ----------------------------------------------------------------- pass --
-//
-// All global accesses of the pass go through buffer resources with 32-bit
-// byte offsets: an offset at or beyond the resource size reads 0 and drops
-// a store, in hardware.  The kernel gives every out-of-grid node (and every
-// lane that does not own its node) the offset kOOB, so the zero boundary of
-// the stencil and the masked stores need no branches.
-
-// voff: per-lane byte offset (kOOB masks the lane); soff: wave-uniform byte
-// offset carried in an SGPR (the level offset), so it costs no VALU add.
-// Either way the range check sees an offset >= kOOB for a masked lane.
-__device__ __forceinline__ float bload(rsrc_t r, uint32_t voff, uint32_t soff, float) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-__device__ __forceinline__ double bload(rsrc_t r, uint32_t voff, uint32_t soff, double) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-  return __hiloint2double((int)v[1], (int)v[0]);
-}
-__device__ __forceinline__ void bstore(float x, rsrc_t r, uint32_t voff, uint32_t soff) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, voff, soff, 0);
-}
-__device__ __forceinline__ void bstore(double x, rsrc_t r, uint32_t voff, uint32_t soff) {
-  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
-  u2 v;
-  v[0] = (unsigned)__double2loint(x);
-  v[1] = (unsigned)__double2hiint(x);
-  __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
-}
-
-// The four Adam moments of one node and level, (m_U, v_U, m_V, v_V): one
-// 16-B access in float32, two in float64.
-__device__ __forceinline__ void mload(rsrc_t r, uint32_t voff, uint32_t soff, float (&m)[4]) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) m[i] = __uint_as_float(v[i]);
-}
-__device__ __forceinline__ void mload(rsrc_t r, uint32_t voff, uint32_t soff, double (&m)[4]) {
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * h, soff, 0);
-    m[2 * h] = __hiloint2double((int)v[1], (int)v[0]);
-    m[2 * h + 1] = __hiloint2double((int)v[3], (int)v[2]);
-  }
-}
-__device__ __forceinline__ void mstore(const float (&m)[4], rsrc_t r, uint32_t voff, uint32_t soff) {
-  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-  u4 v;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) v[i] = __float_as_uint(m[i]);
-  __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
-}
-__device__ __forceinline__ void mstore(const double (&m)[4], rsrc_t r, uint32_t voff, uint32_t soff) {
-  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    u4 v;
-    v[0] = (unsigned)__double2loint(m[2 * h]);
-    v[1] = (unsigned)__double2hiint(m[2 * h]);
-    v[2] = (unsigned)__double2loint(m[2 * h + 1]);
-    v[3] = (unsigned)__double2hiint(m[2 * h + 1]);
-    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff + 16 * h, soff, 0);
-  }
-}
-
-template <typename T>
-struct PassArgs {
-  const T* U;
-  const T* V;
-  T* gU;
-  T* gV;
-  T* Uo;
-  T* Vo;
-  T* mom;         // (nbatch, Nz, Ny, Nx, 4) Adam moments, planar layout only
-  const T* coef;
-  const T* hscale;
-  double* partials;
-  const int32_t* stop;
-  // finalize of the previous iteration, done by every block at its start
-  // (prev_partials != nullptr; nbatch 1): the sums of prev_partials give the
-  // previous COST, the EarlyStopping rule is applied to a copy of *es_in, and
-  // the block decides from it whether this iteration runs; block 0 writes the
-  // new state to *es_out (never *es_in, which other blocks are reading), the
-  // energies to fin_out and the COST to cost_hist[prev_iter]
-  const double* prev_partials;
-  const IgmEarlyStop* es_in;
-  IgmEarlyStop* es_out;
-  double* fin_out;
-  double* cost_hist;
-  int prev_iter;
-  uint32_t vel_bytes;   // size of every velocity-shaped resource
-  uint32_t mom_bytes;
-  uint32_t coef_bytes;
-  uint32_t bs_bytes;    // batch stride of the velocity layout, bytes
-  int Ny, Nx, plane, nstrips, nrowblk, blocks_per_batch;
-  T dz[IGM_MAX_NZ];   // layer thickness dzeta_l
-  T l2dz[IGM_MAX_NZ]; // log2(dzeta_l)
-  T idz[IGM_MAX_NZ];  // 1/dzeta_l
-  T wz[IGM_MAX_NZ];   // level weight of the gravity term: (dzeta_{k-1} + dzeta_k)/2
-  T e_glen;           // (p-2)/2
-  T min_sr2, max_sr2, regu2;
-  T e_slid;           // s/2 - 1
-  T s_slid;           // s
-  T regu_w2;
-  T alpha, omb1, omb2, eps;  // Adam: alpha, 1-beta1, 1-beta2, epsilon
-};
-
 // ------------------------------------------------------------- finalize --
 //
-// Sum of one batch entry's per-block partials in a fixed order (strided
-// per-thread sums, a wave butterfly, then the 4 waves in order) into
-// out[4] = {mean shear, mean sliding, mean gravity, COST}, then the
-// EarlyStopping rule (iceflow/utils.py:92-111) when es is given.  Run by one
-// 256-thread block.  The solver folds it into the next iteration's pass
-// instead (every block redoes the sum of the previous pass's partials, which
-// the kernel boundary has made visible): a "last block sums" fold inside the
-// same pass would need a device-scope release per block, which on the
-// multi-XCD MI355X writes back the whole L2 (measured 18 us per iteration).
-__device__ __forceinline__ void es_apply(IgmEarlyStop& s, double cost, int iter) {
-  s.iters_done = iter + 1;
-  if (!s.has_best) {
-    s.best_loss = cost;
-    s.has_best = 1;
-  } else {
-    const double rel = (s.best_loss - cost) / fabs(s.best_loss);
-    if (rel > s.relative_min_delta) {
-      s.best_loss = cost;
-      s.wait = 0;
-    } else {
-      s.wait += 1;
-      if (s.wait >= s.patience) {
-        s.stopped = 1;
-        s.stop_iter = iter;
+// Sum of one batch entry's per-block partials (block_sum_partials) into
+// out[4] = {mean shear, mean sliding, mean gravity, COST} for each of the
+// `count` iterations of the pass that wrote them (records of 3 * count
+// doubles per block), then the EarlyStopping rule (iceflow/utils.py:92-111)
+// when es is given.  Run by one 256-thread block.  The solver folds it into
+// the next iteration's pass instead (finalize_previous): a "last block sums"
+// fold inside the same pass would need a device-scope release per block,
+// which on the multi-XCD MI355X writes back the whole L2 (measured 18 us per
+// iteration).
+__device__ void finalize_entry(const double* p, int nblk, int count, double* out,
+                               IgmEarlyStop* es, double* cost_hist, int iter) {
+  IgmEarlyStop st{};
+  if (es != nullptr) st = *es;
+  for (int t = 0; t < count; ++t) {
+    double r[3];
+    block_sum_partials(p + 3 * t, nblk, 3 * count, r);
+    if (threadIdx.x == 0) {
+      const double cost = (r[0] + r[1]) + r[2];  // tf.add_n in energy_components order
+      out[0] = r[0];
+      out[1] = r[1];
+      out[2] = r[2];
+      out[3] = cost;
+      if (es != nullptr && !st.stopped) {
+        if (cost_hist != nullptr) cost_hist[iter + t] = cost;
+        es_apply(st, cost, iter + t);
       }
     }
   }
-}
-
-// (shear, sliding, gravity) sums of one entry's partials, in every thread
-__device__ void block_sum_partials(const double* p, int nblk, double (&r)[3]) {
-  __shared__ double fin[4][3];
-  double s0 = 0, s1 = 0, s2 = 0;
-  for (int i = threadIdx.x; i < nblk; i += 256) {
-    s0 += p[3 * i];
-    s1 += p[3 * i + 1];
-    s2 += p[3 * i + 2];
-  }
-  s0 = wave_sum(s0);
-  s1 = wave_sum(s1);
-  s2 = wave_sum(s2);
-  if ((threadIdx.x & 63) == 0) {
-    fin[threadIdx.x >> 6][0] = s0;
-    fin[threadIdx.x >> 6][1] = s1;
-    fin[threadIdx.x >> 6][2] = s2;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < 3; ++c) r[c] = ((fin[0][c] + fin[1][c]) + fin[2][c]) + fin[3][c];
-}
-
-__device__ void finalize_entry(const double* p, int nblk, double* out, IgmEarlyStop* es,
-                               double* cost_hist, int iter) {
-  double r[3];
-  block_sum_partials(p, nblk, r);
-  if (threadIdx.x == 0) {
-    const double cost = (r[0] + r[1]) + r[2];  // tf.add_n in energy_components order
-    out[0] = r[0];
-    out[1] = r[1];
-    out[2] = r[2];
-    out[3] = cost;
-    if (es != nullptr) {
-      if (cost_hist != nullptr) cost_hist[iter] = cost;
-      IgmEarlyStop st = *es;
-      es_apply(st, cost, iter);
-      *es = st;
-    }
-  }
+  if (threadIdx.x == 0 && es != nullptr) *es = st;
 }
 
 // INTER = 0: planar (Nz, Ny, Nx) with separate U, V buffers;
@@ -305,23 +153,8 @@ energy_pass_kernel(const PassArgs<T> a) {
   double e_sh = 0.0, e_sl = 0.0, e_gr = 0.0;
   bool stopped = (a.stop != nullptr) && (*a.stop != 0);
   if (a.prev_partials != nullptr) {
-    double r[3];
-    block_sum_partials(a.prev_partials, a.blocks_per_batch, r);
-    const double cost = (r[0] + r[1]) + r[2];
-    IgmEarlyStop st = *a.es_in;
-    const bool was = st.stopped != 0;
-    if (!was) es_apply(st, cost, a.prev_iter);
-    if (lblk == 0 && threadIdx.x == 0) {
-      *a.es_out = st;  // (when already stopped: carried over unchanged)
-      if (!was) {
-        a.fin_out[0] = r[0];
-        a.fin_out[1] = r[1];
-        a.fin_out[2] = r[2];
-        a.fin_out[3] = cost;
-        if (a.cost_hist != nullptr) a.cost_hist[a.prev_iter] = cost;
-      }
-    }
-    stopped = stopped || (st.stopped != 0);
+    const bool st = finalize_previous(a, lblk == 0);
+    stopped = stopped || st;
   }
   constexpr uint32_t ES = sizeof(T);
 
@@ -408,42 +241,20 @@ energy_pass_kernel(const PassArgs<T> a) {
       load_row(un, vn, jc + 2);  // next step's upper node row
       // keep the loads above ahead of the sweep (the scheduler would sink them)
       __builtin_amdgcn_sched_barrier(0);
-      const T q = cc[0], w = cc[1], gx = cc[2], gy = cc[3], csl = cc[4], bx = cc[5], by = cc[6];
-      const T whh = w * hh2;
 
       // legacy-Adam update of node (jc, ic), level k, from its gradient
       auto adam = [&](int k, T g_u, T g_v, T (&m)[4]) {
-        m[0] += (g_u - m[0]) * a.omb1;
-        m[1] += (g_u * g_u - m[1]) * a.omb2;
-        m[2] += (g_v - m[2]) * a.omb1;
-        m[3] += (g_v * g_v - m[3]) * a.omb2;
-        bstore(up[k] - (m[0] * a.alpha) * rcp(sqrt_(m[1]) + a.eps), make_rsrc(a.Uo, a.vel_bytes), on,
-               k * kstep);
-        bstore(vp[k] - (m[2] * a.alpha) * rcp(sqrt_(m[3]) + a.eps), make_rsrc(a.Vo, a.vel_bytes), on,
-               k * kstep);
+        T un_, vn_;
+        adam_step(m, g_u, g_v, a.omb1, a.omb2, a.alpha, a.eps, up[k], vp[k], un_, vn_);
+        bstore(un_, make_rsrc(a.Uo, a.vel_bytes), on, k * kstep);
+        bstore(vn_, make_rsrc(a.Vo, a.vel_bytes), on, k * kstep);
         mstore(m, rM, om, k * mstep);
       };
-
-      // Adjoint scatter of level k to the 4 corner nodes, then the output of
-      // node (jc, ic), level k (masked stores).  With A = gs - gx and
-      // B = gs + gx, the corners get A -+ gy (left) and B -+ gy (right); the
-      // right ones arrive from lane l-1: node row jc receives
-      // X - Z, node row jc+1 (carried) X + Z, X = A + B(l-1), Z = gy + gy(l-1).
-      auto scatter = [&](T& carry, T gx_, T gy_, T gs_) -> T {
-        const T A = gs_ - gx_, B = gs_ + gx_;
-        const T X = A + lane_from_left(B);
-        const T Z = gy_ + lane_from_left(gy_);
-        const T g = (carry - Z) + X;
-        carry = X + Z;
-        return g;
-      };
-      auto emit = [&](int k, T gux, T guy, T gsu, T gvx, T gvy, T gsv) {
-        const uint32_t sk = k * kstep;
-        const T g_u = scatter(cu[k], gux, guy, gsu);
-        const T g_v = scatter(cv[k], gvx, gvy, gsv);
+      // output of node (jc, ic), level k (masked stores)
+      auto emit = [&](int k, T g_u, T g_v) {
         if constexpr (MODE == 1) {
-          bstore(g_u, make_rsrc(a.gU, a.vel_bytes), on, sk);
-          bstore(g_v, make_rsrc(a.gV, a.vel_bytes), on, sk);
+          bstore(g_u, make_rsrc(a.gU, a.vel_bytes), on, k * kstep);
+          bstore(g_v, make_rsrc(a.gV, a.vel_bytes), on, k * kstep);
         } else if constexpr (kDeferAdam) {
           gu_[k] = g_u;
           gv_[k] = g_v;
@@ -453,84 +264,7 @@ energy_pass_kernel(const PassArgs<T> a) {
           adam(k, g_u, g_v, m);
         }
       };
-
-      // Level sweep; layer l = k-1 is handled once level k is known.  The
-      // horizontal derivatives are kept in units of 1/(4 dx) (ux = 4 dx *
-      // dU/dx at the level), the vertical ones likewise (q holds damp*dx/Hc);
-      // hh2 = 1/(4 dx)^2 restores the strain rate where it is needed.
-      T p_ux = 0, p_uy = 0, p_vx = 0, p_vy = 0, p_su = 0, p_sv = 0;  // level k-1
-      T g_ux = 0, g_uy = 0, g_vx = 0, g_vy = 0, g_su = 0, g_sv = 0;  // its adjoint
-      T c_sh = 0, c_sl = 0, wsu = 0, wsv = 0;
-#pragma unroll
-      for (int k = 0; k < NZ; ++k) {
-        const T su = up[k] + uc[k], du = uc[k] - up[k];
-        const T sv = vp[k] + vc[k], dv = vc[k] - vp[k];
-        const T su_r = lane_from_right(su), du_r = lane_from_right(du);
-        const T sv_r = lane_from_right(sv), dv_r = lane_from_right(dv);
-        const T ux = su_r - su, uy = du + du_r;
-        const T vx = sv_r - sv, vy = dv + dv_r;
-        const T sU = su + su_r, sV = sv + sv_r;  // 4 x node average
-        T n_ux = 0, n_uy = 0, n_vx = 0, n_vy = 0;
-        T n_su = a.wz[k] * gx, n_sv = a.wz[k] * gy;  // gravity (linear)
-        wsu += a.wz[k] * sU;
-        wsv += a.wz[k] * sV;
-        if (k == 0) {  // Weertman sliding on the basal level
-          const T ub = T(0.25) * sU, vb = T(0.25) * sV;
-          const T c2 = ub * bx + vb * by;
-          const T N = (ub * ub + vb * vb) + a.regu_w2 + c2 * c2;
-          const T t = pow_pos(N, a.e_slid);  // N^(s/2 - 1)
-          c_sl = csl * N * t;
-          const T gs = T(0.25) * csl * a.s_slid * t;
-          n_su += gs * (ub + c2 * bx);
-          n_sv += gs * (vb + c2 * by);
-        }
-        if (k >= 1) {
-          const int l = k - 1;
-          const T dUdx = p_ux + ux, dUdy = p_uy + uy;
-          const T dVdx = p_vx + vx, dVdy = p_vy + vy;
-          const T rz = q * a.idz[l];
-          const T dUdz = (sU - p_su) * rz, dVdz = (sV - p_sv) * rz;
-          const T Dux = dUdx - dUdz * bx, Duy = dUdy - dUdz * by;
-          const T Dvx = dVdx - dVdz * bx, Dvy = dVdy - dVdz * by;
-          const T P = Dvx + Duy;  // 2 Exy
-          const T Ezz = -(Dux + Dvy);
-          // sr2 = hh2 (0.5 (Dux^2 + Dvy^2 + Ezz^2) + Exy^2 + 0.25 (dUdz^2 + dVdz^2))
-          const T t1 = (Dux * Dux + Dvy * Dvy) + Ezz * Ezz;
-          const T t2 = (P * P + dUdz * dUdz) + dVdz * dVdz;
-          const T sr2 = h05 * t1 + h025 * t2;
-          const T sclip = clamp_(sr2, a.min_sr2, a.max_sr2);
-          const bool in_rng = (sclip == sr2);  // the clip passes its gradient inside the range
-          const T sc = sclip + a.regu2;
-          const T wl = scaled_pow(sc, a.e_glen, a.dz[l], a.l2dz[l]);  // dzeta_l sc^e
-          c_sh += wl * sr2;
-          // adjoint of the layer (in the scaled variables):
-          // G = w hh2 wl (1 + e sr2 / sc), zero ratio outside the clip range
-          const T WL = whh * wl;
-          const T er = in_rng ? a.e_glen * sr2 : T(0);
-          const T G = WL + WL * (er * rcp(sc));
-          const T Gh = T(0.5) * G;
-          const T gDux = G * (Dux - Ezz), gDvy = G * (Dvy - Ezz), gDxy = Gh * P;
-          const T gdUdz = Gh * dUdz - gDux * bx - gDxy * by;
-          const T gdVdz = Gh * dVdz - gDxy * bx - gDvy * by;
-          g_ux += gDux;
-          g_uy += gDxy;
-          g_vx += gDxy;
-          g_vy += gDvy;
-          n_ux = gDux;
-          n_uy = gDxy;
-          n_vx = gDxy;
-          n_vy = gDvy;
-          const T gzu = gdUdz * rz, gzv = gdVdz * rz;
-          g_su -= gzu;
-          n_su += gzu;
-          g_sv -= gzv;
-          n_sv += gzv;
-          emit(k - 1, g_ux, g_uy, g_su, g_vx, g_vy, g_sv);
-        }
-        p_ux = ux; p_uy = uy; p_vx = vx; p_vy = vy; p_su = sU; p_sv = sV;
-        g_ux = n_ux; g_uy = n_uy; g_vx = n_vx; g_vy = n_vy; g_su = n_su; g_sv = n_sv;
-      }
-      emit(NZ - 1, g_ux, g_uy, g_su, g_vx, g_vy, g_sv);
+      const CellEnergy<T> e = level_sweep<T, NZ>(a, hh2, h05, h025, up, vp, uc, vc, cc, cu, cv, emit);
 
       if constexpr (kDeferAdam) {
         // keep the updates after the sweep (the scheduler would hoist them
@@ -541,9 +275,9 @@ energy_pass_kernel(const PassArgs<T> a) {
       }
 
       if (emit_row) {
-        e_sh += (double)(w * c_sh);
-        e_sl += (double)c_sl;
-        e_gr += (double)(gx * wsu + gy * wsv);
+        e_sh += (double)e.sh;
+        e_sl += (double)e.sl;
+        e_gr += (double)e.gr;
       }
     };
 
@@ -589,49 +323,17 @@ energy_pass_kernel(const PassArgs<T> a) {
   }
 }
 
-// Waves per launch the row split aims at: 4 per CU of MI355X's 256, one per
-// SIMD, all resident at once, so that every SIMD gets one equally long wave
-// (measured at 1000 x 1000: 1024 waves beat 2048-4096, whose extra halo rows
-// cost more than the latency a second wave would hide).  IGM_ENERGY_WAVES
-// overrides it (for tuning sweeps).
-constexpr int kTargetWaves = 1024;
-constexpr int kMinRows = 4;  // fewer rows per wave would be mostly halo
-
-int target_waves() {
-  static int n = [] {
-    const char* e = getenv("IGM_ENERGY_WAVES");
-    const int r = e ? atoi(e) : kTargetWaves;
-    return (r >= 64 && r <= (1 << 20)) ? r : kTargetWaves;
-  }();
-  return n;
-}
-
-struct PassGeom {
-  int nstrips, nrowblk, blocks;
-};
-inline PassGeom pass_geom(int nbatch, int Ny, int Nx) {
-  PassGeom g;
-  g.nstrips = (Nx + kStripOut - 1) / kStripOut;
-  const int64_t per = (int64_t)g.nstrips * nbatch;
-  int64_t want = (target_waves() + per / 2) / per;
-  const int64_t most = Ny / kMinRows > 1 ? Ny / kMinRows : 1;
-  g.nrowblk = (int)(want < 1 ? 1 : (want > most ? most : want));
-  const int waves = g.nstrips * g.nrowblk;
-  g.blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
-  return g;
-}
-
 // ------------------------------------------------------------- finalize --
 
 constexpr int kFinBlock = 256;
 
 __global__ void __launch_bounds__(kFinBlock)
-energy_finalize_kernel(const double* __restrict__ partials, int blocks_per_batch,
+energy_finalize_kernel(const double* __restrict__ partials, int blocks_per_batch, int count,
                        double* __restrict__ out, IgmEarlyStop* es, double* cost_hist, int iter) {
   const int b = blockIdx.x;
   if (es != nullptr && es->stopped) return;
-  finalize_entry(partials + 3 * (int64_t)b * blocks_per_batch, blocks_per_batch, out + 4 * b, es,
-                 cost_hist, iter);
+  finalize_entry(partials + 3 * count * (int64_t)b * blocks_per_batch, blocks_per_batch, count,
+                 out + 4 * b, es, cost_hist, iter);
 }
 
 // ------------------------------------------------------ velocity outputs --
@@ -734,31 +436,11 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
   a.bs_bytes = (uint32_t)((nbatch > 1 ? bs : 0) * (int64_t)sizeof(T));
   a.plane = (int)plane;
   a.Ny = Ny; a.Nx = Nx;
-  const PassGeom g = pass_geom(nbatch, Ny, Nx);
+  const PassGeom g = pass_geom(nbatch, Ny, Nx, kStripOut);
   a.nstrips = g.nstrips; a.nrowblk = g.nrowblk; a.blocks_per_batch = g.blocks;
-  for (int k = 0; k < IGM_MAX_NZ; ++k) { a.dz[k] = a.idz[k] = a.wz[k] = a.l2dz[k] = T(0); }
-  for (int l = 0; l < nz - 1; ++l) {
-    IGM_CHECK_ARG(dzeta[l] > 0.0, "dzeta must be positive");
-    a.dz[l] = T(dzeta[l]);
-    a.idz[l] = T(1.0 / dzeta[l]);
-    a.l2dz[l] = T(std::log2(dzeta[l]));
-  }
-  for (int k = 0; k < nz; ++k) {
-    const double lo = k > 0 ? dzeta[k - 1] : 0.0;
-    const double hi = k < nz - 1 ? dzeta[k] : 0.0;
-    a.wz[k] = T(0.5 * (lo + hi));
-  }
-  const double p = 1.0 + 1.0 / phys->exp_glen;
-  const double s = 1.0 + 1.0 / phys->exp_weertman;
-  a.e_glen = T((p - 2.0) / 2.0);
-  // The clip bounds min_sr^2, max_sr^2 live in the working precision
-  // (cost_shear.py:172): in float32, 1e40 becomes +inf.
-  a.min_sr2 = T(phys->min_sr * phys->min_sr);
-  a.max_sr2 = T(phys->max_sr * phys->max_sr);
-  a.regu2 = T(phys->regu_glen * phys->regu_glen);
-  a.e_slid = T(s / 2.0 - 1.0);
-  a.s_slid = T(s);
-  a.regu_w2 = T(phys->regu_weertman * phys->regu_weertman);
+  a.prev_count = 1;
+  a.prev_nblk = g.blocks;
+  if (const int rc = fill_physics(a, nz, dzeta, phys)) return rc;
   if (adam) {
     a.alpha = T(adam->alpha);
     a.omb1 = T(1) - T(adam->beta1);
@@ -852,8 +534,12 @@ const char* igm_last_error(void) { return igm::last_error(); }
 
 int64_t igm_energy_workspace_doubles(int32_t nbatch, int32_t Ny, int32_t Nx) {
   if (nbatch < 1 || Ny < 2 || Nx < 2) return 0;
-  const igm::PassGeom g = igm::pass_geom(nbatch, Ny, Nx);
-  return 3 * (int64_t)nbatch * g.blocks;
+  // room for either pass: 3 doubles per block of the one-iteration pass,
+  // 6 per block of the two-iteration pass (one field)
+  const igm::PassGeom g = igm::pass_geom(nbatch, Ny, Nx, igm::kStripOut);
+  const igm::PassGeom g2 = igm::pass_geom(1, Ny, Nx, igm::kPairOut);
+  const int64_t one = 3 * (int64_t)nbatch * g.blocks, two = 6 * (int64_t)g2.blocks;
+  return one > two ? one : two;
 }
 
 int igm_energy_coeffs_f32(const float* thk, const float* usurf, const float* arrhenius,
@@ -908,9 +594,20 @@ int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int3
   IGM_CHECK_ARG(partials && out, "null pointer");
   IGM_CHECK_ARG(nbatch >= 1 && Ny >= 2 && Nx >= 2, "bad sizes");
   IGM_CHECK_ARG(es == nullptr || nbatch == 1, "early stopping needs nbatch == 1");
-  const igm::PassGeom g = igm::pass_geom(nbatch, Ny, Nx);
+  const igm::PassGeom g = igm::pass_geom(nbatch, Ny, Nx, igm::kStripOut);
   hipLaunchKernelGGL(igm::energy_finalize_kernel, dim3(nbatch), dim3(igm::kFinBlock), 0,
-                     (hipStream_t)stream, partials, g.blocks, out, es, cost_hist, iter);
+                     (hipStream_t)stream, partials, g.blocks, 1, out, es, cost_hist, iter);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_energy_finalize_pair(const double* partials, int32_t Ny, int32_t Nx, double* out,
+                             IgmEarlyStop* es, double* cost_hist, int32_t iter, void* stream) {
+  IGM_CHECK_ARG(partials && out, "null pointer");
+  IGM_CHECK_ARG(Ny >= 2 && Nx >= 2, "bad sizes");
+  const igm::PassGeom g = igm::pass_geom(1, Ny, Nx, igm::kPairOut);
+  hipLaunchKernelGGL(igm::energy_finalize_kernel, dim3(1), dim3(igm::kFinBlock), 0,
+                     (hipStream_t)stream, partials, g.blocks, 2, out, es, cost_hist, iter);
   IGM_CHECK_LAUNCH();
   return 0;
 }

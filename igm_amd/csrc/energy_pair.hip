@@ -1,0 +1,314 @@
+// Two iterations of IGM's energy solver in one pass on gfx950: energy,
+// adjoint and legacy-Adam update, twice, with temporal blocking of the
+// column march (float32, planar layout, one field).
+//
+// Reference: igm/processes/iceflow/solve/solve.py:45-84 -- two consecutive
+// iterations of the loop, with no EarlyStopping check between them -- and
+// energy/cost_shear.py:103-179, cost_gravity.py:24-69,
+// cost_sliding_weertman.py:23-43.
+//
+// The one-iteration pass reads U, V, the Adam moments and the coefficients
+// and writes U, V and the moments: about 500 B per horizontal cell at Nz 10,
+// which makes an iteration HBM-bound.  This pass moves those bytes once for
+// two iterations.  A wave marches up its rows with two stages one row apart.
+// At step c:
+//   stage A computes cell row c at the input velocity U0, which completes
+//     the gradient of node row c; the first Adam update turns node row c
+//     into U1 and its moments into m1, both kept in registers;
+//   stage B computes cell row c-1 at U1 (node rows c-1 and c, in registers),
+//     which completes node row c-1 for the second iteration; the second Adam
+//     update writes U2 and the moments m2 of that row.
+// U1 and m1 never leave the registers.
+//
+// The price is a wider halo.  U1 is only exact where stage A's gradient is
+// complete, lanes 1..62 of the 64, so stage B's gradient is complete on lanes
+// 2..61: a wave owns 60 columns.  Vertically, stage A starts two rows below
+// the wave's first node row and ends one above its last, so a wave reads node
+// rows j0-2 .. j1+1 of U0 and the moments of rows j0-1 .. j1.  Because
+// neighbouring waves read moment rows that this pass writes, the moments are
+// read from one buffer and written to another.
+//
+// Both stages run the level sweep of the one-iteration pass (level_sweep)
+// with the same operands, so a pair pass gives the same results as two
+// one-iteration passes, bit for bit.
+#include "energy_pass.hpp"
+
+namespace igm {
+
+template <int NZ>
+__global__ void __launch_bounds__(kPassBlock)
+energy_pair_kernel(const PassArgs<float> a) {
+  using T = float;
+  constexpr uint32_t ES = sizeof(T);
+  const int lane = threadIdx.x & (kWave - 1);
+  // logical block: XCD-contiguous bands of (strip, row block) tiles
+  const int lblk = xcd_remap(blockIdx.x, gridDim.x);
+  const int wid = lblk * kWavesPerBlock + (threadIdx.x >> 6);
+  double e1_sh = 0.0, e1_sl = 0.0, e1_gr = 0.0;  // energy at U0 (first iteration)
+  double e2_sh = 0.0, e2_sl = 0.0, e2_gr = 0.0;  // energy at U1 (second iteration)
+  bool stopped = (a.stop != nullptr) && (*a.stop != 0);
+  if (a.prev_partials != nullptr) {
+    const bool st = finalize_previous(a, lblk == 0);
+    stopped = stopped || st;
+  }
+
+  if (!stopped && wid < a.nstrips * a.nrowblk) {
+    const int strip = wid % a.nstrips;
+    const int rblk = wid / a.nstrips;
+    const int ic = strip * kPairOut - 2 + lane;
+    const bool col_ok = (ic >= 0) && (ic < a.Nx);
+    const bool own = (lane >= 2) && (lane <= kPairOut + 1) && col_ok;
+    // node rows [j0, j1) of this wave: the Ny rows split evenly over nrowblk
+    const int j0 = (int)(((int64_t)rblk * a.Ny) / a.nrowblk);
+    const int j1 = (int)(((int64_t)(rblk + 1) * a.Ny) / a.nrowblk);
+    const rsrc_t rU = make_rsrc(a.U, a.vel_bytes), rV = make_rsrc(a.V, a.vel_bytes);
+    const rsrc_t rUo = make_rsrc(a.Uo, a.vel_bytes), rVo = make_rsrc(a.Vo, a.vel_bytes);
+    const rsrc_t rC = make_rsrc(a.coef, a.coef_bytes);
+    const rsrc_t rMi = make_rsrc(a.mom, a.mom_bytes), rMo = make_rsrc(a.mom_out, a.mom_bytes);
+    const uint32_t kstep = (uint32_t)a.plane * ES;  // bytes between levels
+    const uint32_t mstep = 4u * kstep;              // moments: 4 slots per node and level
+    const T hh = a.hscale[0];
+    const T hh2 = hh * hh;
+    const T h05 = T(0.5) * hh2, h025 = T(0.25) * hh2;
+
+    // byte offset of node (row, ic), level 0 -- or kOOB outside the grid
+    auto node_off = [&](int row, bool extra) -> uint32_t {
+      const bool ok = extra && col_ok && row >= 0 && row < a.Ny;
+      return ok ? (uint32_t)(row * a.Nx + ic) * ES : kOOB;
+    };
+    // node row `row` of U0 and V0 (stage A reads rows j0-2 .. j1+1)
+    auto load_row = [&](T (&u)[NZ], T (&v)[NZ], int row) {
+      const uint32_t o = node_off(row, row <= j1 + 1);
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) {
+        u[k] = bload(rU, o, k * kstep, T());
+        v[k] = bload(rV, o, k * kstep, T());
+      }
+    };
+    // coefficient planes of cell row jc (stage A reads rows j0-2 .. j1)
+    const uint32_t ps = (uint32_t)a.plane * ES;
+    auto load_coef = [&](T (&c)[kPlanes], int jc) {
+      const uint32_t co =
+          (col_ok && jc >= 0 && jc < a.Ny && jc <= j1) ? (uint32_t)(jc * a.Nx + ic) * ES : kOOB;
+#pragma unroll
+      for (int p = 0; p < kPlanes; ++p) c[p] = bload(rC, co, p * ps, T());
+    };
+    // moments of node row `row` (rows j0-1 .. j1 give U1; others read 0)
+    auto load_mom = [&](T (&m)[NZ][4], int row) {
+      const uint32_t o = node_off(row, row >= j0 - 1 && row <= j1);
+      const uint32_t om = o == kOOB ? kOOB : 4u * o;
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) mload(rMi, om, k * mstep, m[k]);
+    };
+
+    // upper-corner adjoint contributions of the previous cell row, per stage
+    T cuA[NZ], cvA[NZ], cuB[NZ], cvB[NZ];
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      cuA[k] = cvA[k] = cuB[k] = cvB[k] = T(0);
+    }
+
+    // One step c.  In registers on entry: U0 node rows c (a_lo) and c+1
+    // (a_hi), the coefficients of cell rows c (k_cur) and c-1 (k_prv), U1 of
+    // node row c-1 (q_lo) and its moments m1 (m_hld).  The step issues the
+    // next step's loads first (vector memory returns in order, so the sweeps
+    // never wait on them): the moments of node row c (m_ld, needed at the
+    // end of stage A), the coefficients of cell row c+1 and U0 of node row
+    // c+2.  Stage A leaves U1 of node row c in q_new and its moments in m_ld.
+    auto step = [&](int c, const T (&a_lo_u)[NZ], const T (&a_lo_v)[NZ], const T (&a_hi_u)[NZ],
+                    const T (&a_hi_v)[NZ], T (&a_nx_u)[NZ], T (&a_nx_v)[NZ],
+                    const T (&k_cur)[kPlanes], T (&k_nx)[kPlanes], const T (&k_prv)[kPlanes],
+                    T (&m_ld)[NZ][4], T (&m_hld)[NZ][4], const T (&q_lo_u)[NZ],
+                    const T (&q_lo_v)[NZ], T (&q_new_u)[NZ], T (&q_new_v)[NZ]) {
+      load_mom(m_ld, c);
+      load_coef(k_nx, c + 1);
+      load_row(a_nx_u, a_nx_v, c + 2);
+      // keep the loads above ahead of the sweeps (the scheduler would sink them)
+      __builtin_amdgcn_sched_barrier(0);
+
+      // stage A: cell row c at U0 -> gradient of node row c
+      T gu[NZ], gv[NZ];
+      const CellEnergy<T> eA = level_sweep<T, NZ>(
+          a, hh2, h05, h025, a_lo_u, a_lo_v, a_hi_u, a_hi_v, k_cur, cuA, cvA,
+          [&](int k, T g_u, T g_v) {
+            gu[k] = g_u;
+            gv[k] = g_v;
+          });
+      // keep the first update after the sweep (it waits for the moments)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) {
+        adam_step(m_ld[k], gu[k], gv[k], a.omb1, a.omb2, a.alpha, a.eps, a_lo_u[k], a_lo_v[k],
+                  q_new_u[k], q_new_v[k]);
+      }
+
+      // stage B: cell row c-1 at U1 -> gradient of node row c-1, second
+      // update, stored (owned node rows only)
+      const bool emit = own && (c - 1 >= j0);
+      const uint32_t on = node_off(c - 1, emit);
+      const uint32_t om = on == kOOB ? kOOB : 4u * on;
+      const CellEnergy<T> eB = level_sweep<T, NZ>(
+          a, hh2, h05, h025, q_lo_u, q_lo_v, q_new_u, q_new_v, k_prv, cuB, cvB,
+          [&](int k, T g_u, T g_v) {
+            T un_, vn_;
+            adam_step(m_hld[k], g_u, g_v, a.omb1, a.omb2, a.alpha2, a.eps, q_lo_u[k], q_lo_v[k],
+                      un_, vn_);
+            bstore(un_, rUo, on, k * kstep);
+            bstore(vn_, rVo, on, k * kstep);
+            mstore(m_hld[k], rMo, om, k * mstep);
+          });
+
+      if (own && c >= j0 && c < j1) {
+        e1_sh += (double)eA.sh;
+        e1_sl += (double)eA.sl;
+        e1_gr += (double)eA.gr;
+      }
+      if (emit) {
+        e2_sh += (double)eB.sh;
+        e2_sl += (double)eB.sl;
+        e2_gr += (double)eB.gr;
+      }
+    };
+
+    // Register sets in rotation with period 3 (the loop is unrolled three
+    // times, so the rotation costs no moves): U0 rows, coefficient rows,
+    // moment rows and U1 rows.  The sets read by the first step's stage B
+    // (cell row j0-3) start at zero: zero coefficients make that cell
+    // contribute nothing, and its node rows are below the owned ones.
+    T A0u[NZ], A0v[NZ], A1u[NZ], A1v[NZ], A2u[NZ], A2v[NZ];
+    T K0[kPlanes], K1[kPlanes], K2[kPlanes];
+    T M0[NZ][4], M1[NZ][4], M2[NZ][4];
+    T Q0u[NZ], Q0v[NZ], Q1u[NZ], Q1v[NZ], Q2u[NZ], Q2v[NZ];
+#pragma unroll
+    for (int p = 0; p < kPlanes; ++p) K2[p] = T(0);
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      Q2u[k] = Q2v[k] = T(0);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) M2[k][s] = T(0);
+    }
+    load_row(A0u, A0v, j0 - 2);
+    load_row(A1u, A1v, j0 - 1);
+    load_coef(K0, j0 - 2);
+    for (int c = j0 - 2; c <= j1; c += 3) {
+      step(c, A0u, A0v, A1u, A1v, A2u, A2v, K0, K1, K2, M0, M2, Q2u, Q2v, Q0u, Q0v);
+      if (c + 1 > j1) break;
+      step(c + 1, A1u, A1v, A2u, A2v, A0u, A0v, K1, K2, K0, M1, M0, Q0u, Q0v, Q1u, Q1v);
+      if (c + 2 > j1) break;
+      step(c + 2, A2u, A2v, A0u, A0v, A1u, A1v, K2, K0, K1, M2, M1, Q1u, Q1v, Q2u, Q2v);
+    }
+  }
+
+  // deterministic block reduction: 6 doubles per block, the three energy
+  // terms of each iteration
+  __shared__ double red[kWavesPerBlock][6];
+  double e[6] = {e1_sh, e1_sl, e1_gr, e2_sh, e2_sl, e2_gr};
+#pragma unroll
+  for (int t = 0; t < 6; ++t) e[t] = wave_sum(e[t]);
+  if (lane == 0) {
+#pragma unroll
+    for (int t = 0; t < 6; ++t) red[threadIdx.x >> 6][t] = e[t];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && !stopped) {
+    double* o = a.partials + 6 * (int64_t)lblk;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      double s = 0;
+#pragma unroll
+      for (int wv = 0; wv < kWavesPerBlock; ++wv) s += red[wv][t];
+      o[t] = s;
+    }
+  }
+}
+
+int pair_impl(int nz, int Ny, int Nx, const float* U, const float* V, const float* coef,
+              const float* hscale, const double* dzeta, const IgmPhysics* phys, float* Uo,
+              float* Vo, const float* mom_in, float* mom_out, const IgmAdam* adam1,
+              const IgmAdam* adam2, double* partials, const int32_t* stop,
+              const double* prev_partials, int prev_count, const IgmEarlyStop* es_in,
+              IgmEarlyStop* es_out, double* fin_out, double* cost_hist, int prev_iter,
+              void* stream) {
+  IGM_CHECK_ARG(U && V && coef && hscale && dzeta && phys && partials, "null pointer");
+  IGM_CHECK_ARG(Uo && Vo && mom_in && mom_out && adam1 && adam2,
+                "needs Uout, Vout, both moment buffers and both Adam steps");
+  IGM_CHECK_ARG(Uo != U && Vo != V && Uo != V && Vo != U, "Uout/Vout must not alias U/V");
+  IGM_CHECK_ARG(mom_out != mom_in, "the moments are read from one buffer and written to another");
+  IGM_CHECK_ARG(Ny >= 2 && Nx >= 2, "need a grid of at least 2 x 2");
+  IGM_CHECK_ARG(nz >= 2 && nz <= IGM_MAX_NZ, "nz out of range");
+  IGM_CHECK_ARG(adam1->beta1 == adam2->beta1 && adam1->beta2 == adam2->beta2 &&
+                    adam1->epsilon == adam2->epsilon,
+                "both iterations must share beta1, beta2 and epsilon");
+  const int64_t plane = (int64_t)Ny * Nx;
+  const int64_t vel_bytes = plane * nz * (int64_t)sizeof(float);
+  const int64_t coef_bytes = kPlanes * plane * (int64_t)sizeof(float);
+  const int64_t mom_bytes = 4 * vel_bytes;
+  IGM_CHECK_ARG(mom_bytes < (int64_t)kOOB && coef_bytes < (int64_t)kOOB,
+                "problem too large for 32-bit buffer offsets (2 GiB per buffer)");
+  PassArgs<float> a{};
+  a.U = U; a.V = V; a.Uo = Uo; a.Vo = Vo;
+  a.mom = const_cast<float*>(mom_in);
+  a.mom_out = mom_out;
+  a.coef = coef; a.hscale = hscale; a.partials = partials; a.stop = stop;
+  const PassGeom g = pass_geom(1, Ny, Nx, kPairOut);
+  if (prev_partials) {
+    IGM_CHECK_ARG(prev_count == 1 || prev_count == 2,
+                  "prev_count must be 1 (one-iteration pass) or 2 (two-iteration pass)");
+    IGM_CHECK_ARG(es_in && es_out && fin_out && es_in != es_out,
+                  "fused finalize needs es_in != es_out and out");
+    IGM_CHECK_ARG(prev_partials != partials, "prev_partials must not alias partials");
+    a.prev_count = prev_count;
+    a.prev_nblk = prev_count == 1 ? pass_geom(1, Ny, Nx, kStripOut).blocks : g.blocks;
+  }
+  a.prev_partials = prev_partials; a.es_in = es_in; a.es_out = es_out; a.fin_out = fin_out;
+  a.cost_hist = cost_hist; a.prev_iter = prev_iter;
+  a.vel_bytes = (uint32_t)vel_bytes;
+  a.coef_bytes = (uint32_t)coef_bytes;
+  a.mom_bytes = (uint32_t)mom_bytes;
+  a.plane = (int)plane;
+  a.Ny = Ny; a.Nx = Nx;
+  a.nstrips = g.nstrips; a.nrowblk = g.nrowblk; a.blocks_per_batch = g.blocks;
+  if (const int rc = fill_physics(a, nz, dzeta, phys)) return rc;
+  a.alpha = float(adam1->alpha);
+  a.alpha2 = float(adam2->alpha);
+  a.omb1 = 1.0f - float(adam1->beta1);
+  a.omb2 = 1.0f - float(adam1->beta2);
+  a.eps = float(adam1->epsilon);
+  hipStream_t st = (hipStream_t)stream;
+  switch (nz) {
+#define IGM_PAIR_CASE(N) \
+  case N: hipLaunchKernelGGL((energy_pair_kernel<N>), dim3(g.blocks), dim3(kPassBlock), 0, st, a); break;
+    IGM_PAIR_CASE(2)
+    IGM_PAIR_CASE(3)
+    IGM_PAIR_CASE(4)
+    IGM_PAIR_CASE(5)
+    IGM_PAIR_CASE(6)
+    IGM_PAIR_CASE(8)
+    IGM_PAIR_CASE(10)
+#undef IGM_PAIR_CASE
+    default:
+      set_error("igm_energy_pass2_f32: nz not compiled (supported: 2,3,4,5,6,8,10)");
+      return -1;
+  }
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // namespace igm
+
+extern "C" {
+
+int igm_energy_pass2_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, const float* V,
+                         const float* coef, const float* hscale, const double* dzeta,
+                         const IgmPhysics* phys, float* Uout, float* Vout,
+                         const float* moments_in, float* moments_out, const IgmAdam* adam1,
+                         const IgmAdam* adam2, double* partials, const int32_t* stop_flag,
+                         const double* prev_partials, int32_t prev_count,
+                         const IgmEarlyStop* es_in, IgmEarlyStop* es_out, double* out,
+                         double* cost_hist, int32_t prev_iter, void* stream) {
+  return igm::pair_impl(nz, Ny, Nx, U, V, coef, hscale, dzeta, phys, Uout, Vout, moments_in,
+                        moments_out, adam1, adam2, partials, stop_flag, prev_partials, prev_count,
+                        es_in, es_out, out, cost_hist, prev_iter, stream);
+}
+
+}  // extern "C"

@@ -121,6 +121,40 @@ class EnergyOperator:
                  if prev is not None else (None, None, None, None, None, 0)),
                L.stream_ptr())
 
+    # Nz with a two-iteration pass (igm_energy_pass2_f32)
+    PAIR_NZ = (2, 3, 4, 5, 6, 8, 10)
+
+    @property
+    def pair_supported(self):
+        """Whether ``run_pass2`` serves this operator: float32, one field and
+        an Nz the two-iteration pass is compiled for."""
+        return self.dtype == torch.float32 and self.B == 1 and self.nz in self.PAIR_NZ
+
+    def run_pass2(self, U, V, Uo, Vo, mom_in, mom_out, adam1, adam2, partials, stop_flag=None,
+                  prev=None):
+        """Two solver iterations in one pass (planar (Nz, Ny, Nx) float32):
+        energy, gradient and legacy-Adam step with ``adam1``, then again with
+        ``adam2``.  Moments (Nz, Ny, Nx, 4) are read from ``mom_in`` and
+        written to ``mom_out``.  ``partials`` receives 6 doubles per block.
+        ``prev``: (prev_partials, prev_count, es_in, es_out, cost_hist,
+        prev_iter) -- the previous pass's finalize, done by this pass."""
+        assert self.pair_supported and U.dim() == 3 and U.is_contiguous() and V.is_contiguous()
+        L.call("igm_energy_pass2_f32", self.nz, self.Ny, self.Nx, L.ptr(U), L.ptr(V),
+               L.ptr(self.coef), L.ptr(self.hscale), C.addressof(self.dzeta), C.addressof(self.phys),
+               L.ptr(Uo), L.ptr(Vo), L.ptr(mom_in), L.ptr(mom_out), C.addressof(adam1),
+               C.addressof(adam2), L.ptr(partials), stop_flag,
+               *((L.ptr(prev[0]), prev[1], prev[2], prev[3], L.ptr(self.out), L.ptr(prev[4]), prev[5])
+                 if prev is not None else (None, 0, None, None, None, None, 0)),
+               L.stream_ptr())
+
+    def finalize_pair(self, es=None, cost_hist=None, it=0, partials=None, out=None):
+        """``finalize`` of a ``run_pass2`` call: iterations it and it+1."""
+        out = self.out if out is None else out
+        p = self.partials if partials is None else partials
+        L.call("igm_energy_finalize_pair", L.ptr(p), self.Ny, self.Nx, L.ptr(out), es,
+               L.ptr(cost_hist), it, L.stream_ptr())
+        return out
+
     def finalize(self, out=None, es=None, cost_hist=None, it=0, partials=None):
         out = self.out if out is None else out
         p = self.partials if partials is None else partials

@@ -49,13 +49,18 @@ class SolveResult:
 
 
 def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early_stopping=True,
-          relative_min_delta=0.0002, patience=10, check_every=16, lr_schedule=None):
+          relative_min_delta=0.0002, patience=10, check_every=16, lr_schedule=None, pairs=True):
     """Minimise the energy of ``op`` from (U0, V0) (planar (Nz, Ny, Nx)).
 
     Returns SolveResult with unmasked U, V (device), the COST history (host
     float64 numpy array) and the number of iterations whose update was
     applied.  ``lr_schedule(i)`` (optional) overrides the learning rate of
     iteration i (the reference uses a constant step_size).
+
+    Without early stopping there is no decision between iterations, and the
+    float32 solver runs two iterations per kernel launch (``run_pass2``,
+    which gives the same result as two one-iteration passes); ``pairs=False``
+    keeps one iteration per launch.
     """
     dt = op.dtype
     assert U0.dtype == dt and U0.shape == V0.shape and U0.dim() == 3
@@ -82,11 +87,56 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
                      epsilon=optimizer.epsilon)
     t0 = optimizer.iterations
     lr0 = optimizer.lr
-    n_launched = 0
-    for i in range(nbitmax):
+
+    def alpha_of(i):
         if lr_schedule is not None:
             optimizer.lr = lr_schedule(i)
-        adam.alpha = optimizer.alpha(t0 + i + 1, dt)
+        return optimizer.alpha(t0 + i + 1, dt)
+
+    if pairs and not early_stopping and op.pair_supported and nbitmax >= 2:
+        # Pass k covers one iteration (only the first pass, when nbitmax is
+        # odd) or two; it finalizes the previous pass's iterations itself,
+        # with the same double-buffered states and partials as below.  The
+        # moments ping-pong between two buffers.
+        M = [mom, torch.empty_like(mom)]
+        adam2 = L.IgmAdam(alpha=0.0, beta1=optimizer.beta1, beta2=optimizer.beta2,
+                          epsilon=optimizer.epsilon)
+        i = k = src = cur = 0
+        n_prev = 0
+        while i < nbitmax:
+            if k == 0 and nbitmax % 2 == 1:
+                adam.alpha = alpha_of(i)
+                op.run_pass(2, Ub[src], Vb[src], Uo=Ub[1 - src], Vo=Vb[1 - src], moments=M[cur],
+                            adam=adam, partials=P[k % 2])
+                n_it = 1
+            else:
+                adam.alpha = alpha_of(i)
+                adam2.alpha = alpha_of(i + 1)
+                prev = None if k == 0 else (P[(k - 1) % 2], n_prev, E[k % 2], E[(k - 1) % 2], hist,
+                                            i - n_prev)
+                op.run_pass2(Ub[src], Vb[src], Ub[1 - src], Vb[1 - src], M[cur], M[1 - cur], adam,
+                             adam2, P[k % 2], prev=prev)
+                cur = 1 - cur
+                n_it = 2
+            src = 1 - src
+            i += n_it
+            n_prev = n_it
+            k += 1
+        last = k % 2
+        if n_prev == 1:
+            op.finalize(es=E[last], cost_hist=hist, it=i - 1, partials=P[(k - 1) % 2])
+        else:
+            op.finalize_pair(es=E[last], cost_hist=hist, it=i - 2, partials=P[(k - 1) % 2])
+        optimizer.lr = lr0
+        raw = bytes(es_dev.cpu().numpy().tobytes())
+        st = L.IgmEarlyStop.from_buffer_copy(raw[last * es_size:(last + 1) * es_size])
+        n = int(st.iters_done)
+        optimizer.iterations = t0 + n
+        return SolveResult(Ub[src], Vb[src], hist[:n].cpu().numpy(), n, bool(st.stopped))
+
+    n_launched = 0
+    for i in range(nbitmax):
+        adam.alpha = alpha_of(i)
         src, dst = i % 2, (i + 1) % 2
         prev = None if i == 0 else (P[(i - 1) % 2], E[i % 2], E[(i - 1) % 2], hist, i - 1)
         op.run_pass(2, Ub[src], Vb[src], Uo=Ub[dst], Vo=Vb[dst], moments=mom, adam=adam,

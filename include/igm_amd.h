@@ -68,7 +68,7 @@ const char* igm_last_error(void);
 /* ---------------------------------------------------------------- energy -- */
 
 /* Number of double words of the `partials` workspace that
- * igm_energy_pass_* needs for this problem size. */
+ * igm_energy_pass_* and igm_energy_pass2_f32 need for this problem size. */
 int64_t igm_energy_workspace_doubles(int32_t nbatch, int32_t Ny, int32_t Nx);
 
 /* Per-cell, velocity-independent coefficients of the staggered energy:
@@ -140,6 +140,37 @@ int igm_energy_pass_f64(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, in
 int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int32_t Nx,
                         double* out, IgmEarlyStop* es, double* cost_hist, int32_t iter,
                         void* stream);
+
+/* Two consecutive iterations of solve_iceflow's loop (solve/solve.py:45-84)
+ * in one pass: energy, adjoint and legacy Adam at U (iteration i, step
+ * adam1), then again at the updated field (iteration i+1, step adam2), with
+ * no EarlyStopping check between the two.  Float32, planar (Nz, Ny, Nx), one
+ * field.  The result equals two igm_energy_pass_f32 mode-2 calls.
+ * Writes Uout, Vout (must not alias U, V).  Moments [nz][Ny][Nx][4] are read
+ * from moments_in and written to moments_out (another buffer: neighbouring
+ * waves read rows this pass writes).  adam1, adam2: same beta1, beta2 and
+ * epsilon; alpha may differ.
+ * partials: 6 doubles per block, the (shear, sliding, gravity) sums of
+ * iteration i then i+1 (igm_energy_workspace_doubles() covers them).
+ * prev_partials, es_in, es_out, out, cost_hist, prev_iter: as for
+ * igm_energy_pass_f32; prev_count = 1 when the previous pass was an
+ * igm_energy_pass_f32 call, 2 when it was an igm_energy_pass2_f32 call (its
+ * two iterations prev_iter and prev_iter+1 are accounted in order).
+ * nz: 2, 3, 4, 5, 6, 8 or 10. */
+int igm_energy_pass2_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, const float* V,
+                         const float* coef, const float* hscale, const double* dzeta,
+                         const IgmPhysics* phys, float* Uout, float* Vout,
+                         const float* moments_in, float* moments_out, const IgmAdam* adam1,
+                         const IgmAdam* adam2, double* partials, const int32_t* stop_flag,
+                         const double* prev_partials, int32_t prev_count,
+                         const IgmEarlyStop* es_in, IgmEarlyStop* es_out, double* out,
+                         double* cost_hist, int32_t prev_iter, void* stream);
+
+/* igm_energy_finalize for the partials of an igm_energy_pass2_f32 call:
+ * COST and EarlyStopping (solve.py:57-59, 112) of iterations iter and
+ * iter+1, in order; out[4] holds the energies of the second. */
+int igm_energy_finalize_pair(const double* partials, int32_t Ny, int32_t Nx, double* out,
+                             IgmEarlyStop* es, double* cost_hist, int32_t iter, void* stream);
 
 /* ---------------------------------------------------- velocity outputs -- */
 
