@@ -46,6 +46,10 @@ energy_pair_kernel(const PassArgs<float> a) {
   const int wid = lblk * kWavesPerBlock + (threadIdx.x >> 6);
   double e1_sh = 0.0, e1_sl = 0.0, e1_gr = 0.0;  // energy at U0 (first iteration)
   double e2_sh = 0.0, e2_sl = 0.0, e2_gr = 0.0;  // energy at U1 (second iteration)
+  // per wave: two slots of one node row's m1 (16 B) and U1 (8 B) per level
+  // and lane -- 120 KB per block at Nz 10, within gfx950's 160 KB of LDS
+  __shared__ float4 lds_m[kWavesPerBlock][2][NZ][kWave];
+  __shared__ float2 lds_q[kWavesPerBlock][2][NZ][kWave];
   bool stopped = (a.stop != nullptr) && (*a.stop != 0);
   if (a.prev_partials != nullptr) {
     const bool st = finalize_previous(a, lblk == 0);
@@ -108,19 +112,27 @@ energy_pair_kernel(const PassArgs<float> a) {
       cuA[k] = cvA[k] = cuB[k] = cvB[k] = T(0);
     }
 
+    // Stage B's inputs from the previous step -- U1 and the moments m1 of
+    // node row c-1 -- wait in LDS rather than in registers, where they would
+    // be live across the whole of stage A.  Each wave has a private pair of
+    // slots that alternate with the row parity: step c writes row c to slot
+    // c & 1 after its first update, and its stage B reads row c-1 from the
+    // other slot.  LDS accesses of one wave complete in order, so no barrier
+    // is needed.
+    float4 (&sm)[2][NZ][kWave] = lds_m[threadIdx.x >> 6];
+    float2 (&sq)[2][NZ][kWave] = lds_q[threadIdx.x >> 6];
+
     // One step c.  In registers on entry: U0 node rows c (a_lo) and c+1
-    // (a_hi), the coefficients of cell rows c (k_cur) and c-1 (k_prv), U1 of
-    // node row c-1 (q_lo) and its moments m1 (m_hld).  The step issues the
-    // next step's loads first (vector memory returns in order, so the sweeps
-    // never wait on them): the moments of node row c (m_ld, needed at the
-    // end of stage A), the coefficients of cell row c+1 and U0 of node row
-    // c+2.  Stage A leaves U1 of node row c in q_new and its moments in m_ld.
+    // (a_hi) and the coefficients of cell rows c (k_cur) and c-1 (k_prv).
+    // The step issues the next step's loads first (vector memory returns in
+    // order, so the sweeps never wait on them): the moments of node row c
+    // (needed at the end of stage A), the coefficients of cell row c+1 and
+    // U0 of node row c+2.
     auto step = [&](int c, const T (&a_lo_u)[NZ], const T (&a_lo_v)[NZ], const T (&a_hi_u)[NZ],
                     const T (&a_hi_v)[NZ], T (&a_nx_u)[NZ], T (&a_nx_v)[NZ],
-                    const T (&k_cur)[kPlanes], T (&k_nx)[kPlanes], const T (&k_prv)[kPlanes],
-                    T (&m_ld)[NZ][4], T (&m_hld)[NZ][4], const T (&q_lo_u)[NZ],
-                    const T (&q_lo_v)[NZ], T (&q_new_u)[NZ], T (&q_new_v)[NZ]) {
-      load_mom(m_ld, c);
+                    const T (&k_cur)[kPlanes], T (&k_nx)[kPlanes], const T (&k_prv)[kPlanes]) {
+      T m[NZ][4];
+      load_mom(m, c);
       load_coef(k_nx, c + 1);
       load_row(a_nx_u, a_nx_v, c + 2);
       // keep the loads above ahead of the sweeps (the scheduler would sink them)
@@ -136,66 +148,75 @@ energy_pair_kernel(const PassArgs<float> a) {
           });
       // keep the first update after the sweep (it waits for the moments)
       __builtin_amdgcn_sched_barrier(0);
+      // first update: U1 and m1 of node row c, stashed for the next step
+      T q_u[NZ], q_v[NZ];
+      const int sn = c & 1;
 #pragma unroll
       for (int k = 0; k < NZ; ++k) {
-        adam_step(m_ld[k], gu[k], gv[k], a.omb1, a.omb2, a.alpha, a.eps, a_lo_u[k], a_lo_v[k],
-                  q_new_u[k], q_new_v[k]);
+        adam_step(m[k], gu[k], gv[k], a.omb1, a.omb2, a.alpha, a.eps, a_lo_u[k], a_lo_v[k],
+                  q_u[k], q_v[k]);
+        sm[sn][k][lane] = make_float4(m[k][0], m[k][1], m[k][2], m[k][3]);
+        sq[sn][k][lane] = make_float2(q_u[k], q_v[k]);
       }
-
-      // stage B: cell row c-1 at U1 -> gradient of node row c-1, second
-      // update, stored (owned node rows only)
-      const bool emit = own && (c - 1 >= j0);
-      const uint32_t on = node_off(c - 1, emit);
-      const uint32_t om = on == kOOB ? kOOB : 4u * on;
-      const CellEnergy<T> eB = level_sweep<T, NZ>(
-          a, hh2, h05, h025, q_lo_u, q_lo_v, q_new_u, q_new_v, k_prv, cuB, cvB,
-          [&](int k, T g_u, T g_v) {
-            T un_, vn_;
-            adam_step(m_hld[k], g_u, g_v, a.omb1, a.omb2, a.alpha2, a.eps, q_lo_u[k], q_lo_v[k],
-                      un_, vn_);
-            bstore(un_, rUo, on, k * kstep);
-            bstore(vn_, rVo, on, k * kstep);
-            mstore(m_hld[k], rMo, om, k * mstep);
-          });
 
       if (own && c >= j0 && c < j1) {
         e1_sh += (double)eA.sh;
         e1_sl += (double)eA.sl;
         e1_gr += (double)eA.gr;
       }
-      if (emit) {
-        e2_sh += (double)eB.sh;
-        e2_sl += (double)eB.sl;
-        e2_gr += (double)eB.gr;
+
+      // stage B: cell row c-1 at U1 (node rows c-1 from LDS and c) ->
+      // gradient of node row c-1, second update, stored (owned node rows
+      // only).  In the wave's first two steps (cell rows j0-3 and j0-2) it
+      // would only feed node rows below the wave's own, so the wave skips it
+      // (a wave-uniform branch).
+      if (c >= j0) {
+        const int sp = sn ^ 1;
+        T p_u[NZ], p_v[NZ];
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          const float2 t = sq[sp][k][lane];
+          p_u[k] = t.x;
+          p_v[k] = t.y;
+        }
+        const bool emit = own && (c - 1 >= j0);
+        const uint32_t on = node_off(c - 1, emit);
+        const uint32_t om = on == kOOB ? kOOB : 4u * on;
+        const CellEnergy<T> eB = level_sweep<T, NZ>(
+            a, hh2, h05, h025, p_u, p_v, q_u, q_v, k_prv, cuB, cvB, [&](int k, T g_u, T g_v) {
+              const float4 t = sm[sp][k][lane];
+              T mk[4] = {t.x, t.y, t.z, t.w};
+              T un_, vn_;
+              adam_step(mk, g_u, g_v, a.omb1, a.omb2, a.alpha2, a.eps, p_u[k], p_v[k], un_, vn_);
+              bstore(un_, rUo, on, k * kstep);
+              bstore(vn_, rVo, on, k * kstep);
+              mstore(mk, rMo, om, k * mstep);
+            });
+        if (emit) {
+          e2_sh += (double)eB.sh;
+          e2_sl += (double)eB.sl;
+          e2_gr += (double)eB.gr;
+        }
       }
     };
 
-    // Register sets in rotation with period 3 (the loop is unrolled three
-    // times, so the rotation costs no moves): U0 rows, coefficient rows,
-    // moment rows and U1 rows.  The sets read by the first step's stage B
-    // (cell row j0-3) start at zero: zero coefficients make that cell
-    // contribute nothing, and its node rows are below the owned ones.
+    // U0 rows and coefficient rows in rotation with period 3 (the loop is
+    // unrolled three times, so the rotation costs no moves).  The set that
+    // the first unrolled step would read in stage B starts at zero (that
+    // stage B is skipped at c = j0-2).
     T A0u[NZ], A0v[NZ], A1u[NZ], A1v[NZ], A2u[NZ], A2v[NZ];
     T K0[kPlanes], K1[kPlanes], K2[kPlanes];
-    T M0[NZ][4], M1[NZ][4], M2[NZ][4];
-    T Q0u[NZ], Q0v[NZ], Q1u[NZ], Q1v[NZ], Q2u[NZ], Q2v[NZ];
 #pragma unroll
     for (int p = 0; p < kPlanes; ++p) K2[p] = T(0);
-#pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-      Q2u[k] = Q2v[k] = T(0);
-#pragma unroll
-      for (int s = 0; s < 4; ++s) M2[k][s] = T(0);
-    }
     load_row(A0u, A0v, j0 - 2);
     load_row(A1u, A1v, j0 - 1);
     load_coef(K0, j0 - 2);
     for (int c = j0 - 2; c <= j1; c += 3) {
-      step(c, A0u, A0v, A1u, A1v, A2u, A2v, K0, K1, K2, M0, M2, Q2u, Q2v, Q0u, Q0v);
+      step(c, A0u, A0v, A1u, A1v, A2u, A2v, K0, K1, K2);
       if (c + 1 > j1) break;
-      step(c + 1, A1u, A1v, A2u, A2v, A0u, A0v, K1, K2, K0, M1, M0, Q0u, Q0v, Q1u, Q1v);
+      step(c + 1, A1u, A1v, A2u, A2v, A0u, A0v, K1, K2, K0);
       if (c + 2 > j1) break;
-      step(c + 2, A2u, A2v, A0u, A0v, A1u, A1v, K2, K0, K1, M2, M1, Q1u, Q1v, Q2u, Q2v);
+      step(c + 2, A2u, A2v, A0u, A0v, A1u, A1v, K2, K0, K1);
     }
   }
 
