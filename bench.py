@@ -49,11 +49,12 @@ BYTES_TRANSPORT = 28.0
 # `bench.py --kernels-only`, scaled by a 1 GiB copy (the roofline's `traffic`)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_domes1000.json")
 # schedule family searched for both paths: initial step, then total iterations
-LR0S = (30.0, 100.0, 300.0, 1000.0)
-FLATS = (0.1, 0.3)  # fraction of the iterations at the initial step
+LR0S = (100.0, 300.0, 1000.0, 3000.0)
+FLATS = (0.0, 0.1, 0.3)  # fraction of the iterations at the initial step
+ENDS = (1e-3, 1.0, 10.0, 30.0)  # step at the end of the geometric decay
 CPU_SAMPLE_S = 15.0  # wall time of the CPU solver-iteration sample
-REFINE_TOTALS = (100, 200, 300, 500, 700, 1000, 1500, 2000, 3000, 4500, 7000, 10000, 14000)
-SOLVER_TOTALS = (1000, 1500, 2000, 3000, 4500, 7000, 10000, 14000, 20000)
+REFINE_TOTALS = (100, 200, 300, 400, 500, 700, 1000, 1500, 2000, 3000, 4500, 7000, 10000, 14000)
+SOLVER_TOTALS = (200, 300, 400, 500, 700, 1000, 1500, 2000, 3000, 4500, 7000, 10000, 14000, 20000)
 
 
 def log(*a):
@@ -254,23 +255,26 @@ def main():
 
     def schedule_search(U0, V0, totals):
         """Shortest schedule of the family {lr0 in LR0S; a fraction in FLATS
-        at lr0, then a geometric decay to 1e-3} from (U0, V0) that ends
-        <= 1 %: the first total of ``totals`` that does, then bisected (3
-        steps, to 50 iterations) against the total before it.  Returns
-        (its, lr0, err, flat fraction) (the last tried if none reaches 1 %)."""
+        at lr0, then a geometric decay to a step in ENDS} from (U0, V0) that
+        ends <= 1 %: the first total of ``totals`` that does, then bisected
+        (4 steps, to 10 iterations) against the total before it.  Returns
+        (its, lr0, err, flat fraction, end step) (the last tried if none
+        reaches 1 %)."""
 
         def trial(total):
             if total == 0:
-                return (0, None, rel_err(U0, V0, Ur, Vr, mask), None)
+                return (0, None, rel_err(U0, V0, Ur, Vr, mask), None, None)
             cand = []
             for lr0 in LR0S:
                 for ff in FLATS:
-                    n_flat = int(ff * total)
-                    r = solve(op32, U0, V0, total, LegacyAdamState(lr=lr0), early_stopping=False,
-                              lr_schedule=adam_schedule(lr0, n_flat, total - n_flat))
-                    cand.append((total, lr0, rel_err(r.U, r.V, Ur, Vr, mask), ff))
+                    for lr_end in ENDS:
+                        n_flat = int(ff * total)
+                        r = solve(op32, U0, V0, total, LegacyAdamState(lr=lr0), early_stopping=False,
+                                  lr_schedule=adam_schedule(lr0, n_flat, total - n_flat, lr_end))
+                        cand.append((total, lr0, rel_err(r.U, r.V, Ur, Vr, mask), ff, lr_end))
             best = min(cand, key=lambda c: c[2])
-            log(f"[bench]   {total} its: best rel err {best[2]:.4f} (lr0 {best[1]}, flat {best[3]})")
+            log(f"[bench]   {total} its: best rel err {best[2]:.4f} (lr0 {best[1]}, flat {best[3]}, "
+                f"end {best[4]})")
             return best
 
         lo, best = None, None
@@ -282,8 +286,8 @@ def main():
         if best[2] > 0.01 or lo is None:
             return best
         hi = best[0]
-        for _ in range(3):
-            mid = int(round((lo + hi) / 100.0)) * 50
+        for _ in range(4):
+            mid = int(round((lo + hi) / 20.0)) * 10
             if mid <= lo or mid >= hi:
                 break
             t = trial(mid)
@@ -307,11 +311,12 @@ def main():
             EM.update_iceflow_emulated(cfg, state)
     inf_ms = tm.ms / args.steps
     log(f"[bench] emulator inference {inf_ms:.3f} ms, rel err {inf_err:.4f}; searching the polish")
-    n_ref, lr_ref, _, ff_ref = schedule_search(U_emu, V_emu, (0,) + REFINE_TOTALS)
+    n_ref, lr_ref, _, ff_ref, end_ref = schedule_search(U_emu, V_emu, (0,) + REFINE_TOTALS)
     ice.emulator.refine.its = int(n_ref)
     if lr_ref is not None:
         ice.emulator.refine.lr = float(lr_ref)
         ice.emulator.refine.flat_frac = float(ff_ref)
+        ice.emulator.refine.lr_end = float(end_ref)
 
     def emulator_step():
         EM.update_iceflow_emulated(cfg, state)
@@ -348,10 +353,10 @@ def main():
     ref_settings = {"lr": 1.0, "error_curve": curve}
     # (b) the same method and schedule family as the polish, from zero
     log("[bench] searching the solver schedule from zero")
-    n1, lr1, solver_err, ff1 = schedule_search(z, z, SOLVER_TOTALS)
+    n1, lr1, solver_err, ff1, end1 = schedule_search(z, z, SOLVER_TOTALS)
     solver_reached = solver_err <= 0.01
     n_flat1 = int(ff1 * n1)
-    sched = adam_schedule(lr1, n_flat1, n1 - n_flat1)
+    sched = adam_schedule(lr1, n_flat1, n1 - n_flat1, end1)
     solve(op32, z, z.clone(), 20, LegacyAdamState(lr=1.0), early_stopping=False)
     with Timer() as tm:
         rs = solve(op32, z, z.clone(), n1, LegacyAdamState(lr=lr1), early_stopping=False, lr_schedule=sched)
@@ -463,6 +468,7 @@ def main():
                                "<= 1 % rel err vs the float64 minimiser, plus the float64 energy-gap check",
                    "weights": os.path.basename(wpath),
                    "polish_its": int(n_ref), "polish_lr0": lr_ref, "polish_flat_frac": ff_ref,
+                   "polish_lr_end": end_ref,
                    "fallback_check_its": int(ice.emulator.fallback.check_its)},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -471,7 +477,7 @@ def main():
                                "rel_err_vs_fp64": round(inf_err, 5), "energy_gap_vs_fp64": round(gap_emu, 6)},
         "fallbacks": int(state.fallback_count),
         "solver_fields_per_s": round(solver_fps, 4),
-        "solver_schedule": {"lr0": lr1, "flat": int(n_flat1), "decay_to_1e-3": int(n1 - n_flat1),
+        "solver_schedule": {"lr0": lr1, "flat": int(n_flat1), "decay": int(n1 - n_flat1), "lr_end": end1,
                             "rel_err": round(solver_err, 5)},
         "solver_iters_to_1pct": int(n1),
         "solver_reached_1pct": bool(solver_reached),
