@@ -364,14 +364,22 @@ def update_iceflow_emulator(cfg, state, it, pertubate=False):
 
 
 def save_iceflow_model(cfg, state, directory="iceflow-model"):
-    """emulate.py:331-359: weights (safetensors), fieldin.dat, vert_grid.dat."""
+    """emulate.py:331-359: the weights as a Keras 2 HDF5 model file
+    (``model.h5``, which the reference's ``tf.keras.models.load_model`` reads)
+    and as safetensors (``model.safetensors``, this package's own format),
+    ``fieldin.dat`` and ``vert_grid.dat``."""
     import shutil
+    from .keras_h5 import write_keras_cnn
     ice = cfg.processes.iceflow
+    net = ice.emulator.network
     if os.path.exists(directory):
         shutil.rmtree(directory)
     os.mkdir(directory)
     state.iceflow_model.save(os.path.join(directory, "model.safetensors"), list(ice.emulator.fieldin),
                              int(ice.numerics.Nz), float(ice.numerics.vert_spacing))
+    write_keras_cnn(os.path.join(directory, "model.h5"), state.iceflow_model.get_weights(),
+                    activation=net.activation, dropout_rate=float(net.dropout_rate),
+                    weight_initialization=net.weight_initialization)
     with open(os.path.join(directory, "fieldin.dat"), "w") as fid:
         for key in ice.emulator.fieldin:
             fid.write("%s \n" % (key))

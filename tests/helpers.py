@@ -42,3 +42,56 @@ def rel(a, b):
     a = torch.as_tensor(a, dtype=torch.float64).cpu()
     b = torch.as_tensor(b, dtype=torch.float64).cpu()
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def h5_structure(path):
+    """Groups, attributes (type, shape, value) and datasets (type, shape) of an
+    HDF5 file, from h5dump's XML dump of its header and attributes (a data
+    dump: nothing in the file is executed).  String values are unescaped; a
+    ``model_config`` attribute is parsed as JSON.  Dataset values are not read."""
+    import json
+    import re
+    import subprocess
+    import xml.etree.ElementTree as ET
+    from igm_amd.keras_h5 import _h5dump
+    xml = subprocess.run([_h5dump(), "-x", "-H", "-A", path], capture_output=True, text=True,
+                         check=True).stdout
+    H = "{http://hdfgroup.org/HDF5/XML/schema/HDF5-File.xsd}"
+
+    def dtype(node):
+        st, fl = node.find(f".//{H}StringType"), node.find(f".//{H}FloatType")
+        if st is not None:
+            return f"str/{st.get('Cset')}/{st.get('StrSize')}/{st.get('StrPad')}"
+        if fl is not None:
+            return f"float{8 * int(fl.get('Size'))}/{fl.get('ByteOrder')}"
+        return "other"
+
+    def shape(node):
+        sp = node.find(f"{H}Dataspace")
+        if sp.find(f"{H}ScalarDataspace") is not None:
+            return "scalar"
+        return [int(d.get("DimSize")) for d in sp.iter(f"{H}Dimension")]
+
+    def attribute(a):
+        t, sh = dtype(a), shape(a)
+        out = {"type": t, "shape": sh}
+        if t.startswith("str"):
+            raw = (a.find(f".//{H}DataFromFile").text or "")
+            vals = [re.sub(r"\\(.)", r"\1", m) for m in re.findall(r'"((?:[^"\\]|\\.)*)"', raw)]
+            out["value"] = vals[0] if sh == "scalar" else vals
+            if a.get("Name") == "model_config":
+                out["value"] = json.loads(out["value"])
+        return out
+
+    def group(g):
+        node = {"attrs": {}, "groups": {}, "datasets": {}}
+        for ch in g:
+            if ch.tag == f"{H}Attribute":
+                node["attrs"][ch.get("Name")] = attribute(ch)
+            elif ch.tag == f"{H}Group":
+                node["groups"][ch.get("Name")] = group(ch)
+            elif ch.tag == f"{H}Dataset":
+                node["datasets"][ch.get("Name")] = {"type": dtype(ch), "shape": shape(ch)}
+        return node
+
+    return group(ET.fromstring(xml).find(f"{H}RootGroup"))

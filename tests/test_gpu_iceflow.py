@@ -79,3 +79,35 @@ def test_unsupported_options_raise(dev):
     cfg.processes.iceflow.numerics.vert_basis = "Chebyshev"
     with pytest.raises(ValueError):
         iceflow.initialize(cfg, st)
+
+
+def test_save_model_writes_a_keras_directory_that_loads_back(dev, tmp_path, monkeypatch):
+    """finalize with save_model (emulate.py:331-359) writes iceflow-model/
+    with model.h5 (Keras 2 HDF5), fieldin.dat and vert_grid.dat; pointing
+    emulator.name at that directory loads the same weights."""
+    monkeypatch.chdir(tmp_path)
+    cfg = config.default_cfg()
+    cfg.processes.iceflow.emulator.save_model = True
+    g = syn.dome100()
+    state = State()
+    for k in ("thk", "usurf", "dX"):
+        setattr(state, k, g[k])
+    state.it = -1
+    iceflow.initialize(cfg, state)
+    iceflow.finalize(cfg, state)
+    d = tmp_path / "iceflow-model"
+    assert sorted(p.name for p in d.iterdir()) == ["fieldin.dat", "model.h5", "model.safetensors",
+                                                   "vert_grid.dat"]
+    w0 = state.iceflow_model.get_weights()
+    cfg2 = config.default_cfg()
+    cfg2.processes.iceflow.emulator.name = str(d)
+    cfg2.processes.iceflow.emulator.nbit_init = 0
+    state2 = State()
+    for k in ("thk", "usurf", "dX"):
+        setattr(state2, k, g[k])
+    state2.it = -1
+    iceflow.initialize(cfg2, state2)
+    w1 = state2.iceflow_model.get_weights()
+    assert len(w0) == len(w1)
+    for (k0, b0), (k1, b1) in zip(w0, w1):
+        assert np.array_equal(k0, k1) and np.array_equal(b0, b1)
