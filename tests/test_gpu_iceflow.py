@@ -101,7 +101,7 @@ def test_save_model_writes_a_keras_directory_that_loads_back(dev, tmp_path, monk
     w0 = state.iceflow_model.get_weights()
     cfg2 = config.default_cfg()
     cfg2.processes.iceflow.emulator.name = str(d)
-    cfg2.processes.iceflow.emulator.nbit_init = 0
+    cfg2.processes.iceflow.emulator.retrain_freq = 0  # no retraining step at initialize
     state2 = State()
     for k in ("thk", "usurf", "dX"):
         setattr(state2, k, g[k])
