@@ -35,6 +35,7 @@ class IgmEarlyStop(C.Structure):
     _fields_ = [
         ("best_loss", F64), ("relative_min_delta", F64), ("has_best", I32), ("wait", I32),
         ("patience", I32), ("stopped", I32), ("stop_iter", I32), ("iters_done", I32),
+        ("passes", I32),
     ]
 
 
@@ -54,6 +55,8 @@ _SIGS = {
     "igm_energy_pass2_f32": (C.c_int, [I32, I32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
                                        VP, VP, I32, VP, VP, VP, VP, I32, VP]),
     "igm_energy_finalize_pair": (C.c_int, [VP, I32, I32, VP, VP, VP, I32, VP]),
+    "igm_energy_solve_pass_f32": (C.c_int, [I32, I32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
+                                            I32, VP, VP, VP, VP, VP, VP, I32, VP]),
     "igm_velocity_outputs_f32": (C.c_int, [I32, I32, I32, VP, VP, I64, I64, I64, VP, VP, VP, VP, VP, VP, VP,
                                            VP, VP, VP, VP]),
     "igm_velocity_outputs_f64": (C.c_int, [I32, I32, I32, VP, VP, I64, I64, I64, VP, VP, VP, VP, VP, VP, VP,

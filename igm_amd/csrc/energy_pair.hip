@@ -35,6 +35,37 @@
 
 namespace igm {
 
+// Prologue of a device-driven solve pass (a.alpha_tab set), done by every
+// block.  The state *es_in is the one the previous pass started from, so the
+// previous pass's iteration count is recomputed from it exactly as that pass
+// decided it; those iterations are then accounted in order (their COSTs from
+// the previous partials, the EarlyStopping rule of iceflow/utils.py:92-111
+// on a private copy).  Block 0 writes the new state to *es_out and the COSTs
+// to cost_hist.  Returns the iterations this pass runs (0, 1 or 2) and its
+// first iteration in *it0.
+__device__ int solve_prologue(const PassArgs<float>& a, bool block0, int* it0) {
+  IgmEarlyStop st = *a.es_in;
+  const int f_prev = st.iters_done;
+  const int c_prev = a.prev_partials != nullptr ? pass_iterations(st, f_prev, a.nbitmax) : 0;
+  for (int t = 0; t < c_prev; ++t) {
+    double r[3];
+    block_sum_partials(a.prev_partials + 3 * t, a.prev_nblk, 6, r);
+    const double cost = (r[0] + r[1]) + r[2];
+    es_apply(st, cost, f_prev + t);
+    if (block0 && threadIdx.x == 0) {
+      a.fin_out[0] = r[0];
+      a.fin_out[1] = r[1];
+      a.fin_out[2] = r[2];
+      a.fin_out[3] = cost;
+      if (a.cost_hist != nullptr) a.cost_hist[f_prev + t] = cost;
+    }
+  }
+  if (c_prev > 0) st.passes += 1;
+  if (block0 && threadIdx.x == 0) *a.es_out = st;
+  *it0 = st.iters_done;
+  return a.finalize_only ? 0 : pass_iterations(st, st.iters_done, a.nbitmax);
+}
+
 template <int NZ>
 __global__ void __launch_bounds__(kPassBlock)
 energy_pair_kernel(const PassArgs<float> a) {
@@ -51,7 +82,18 @@ energy_pair_kernel(const PassArgs<float> a) {
   __shared__ float4 lds_m[kWavesPerBlock][2][NZ][kWave];
   __shared__ float2 lds_q[kWavesPerBlock][2][NZ][kWave];
   bool stopped = (a.stop != nullptr) && (*a.stop != 0);
-  if (a.prev_partials != nullptr) {
+  bool pair = true;  // two iterations (else one)
+  float alpha1 = a.alpha, alpha2 = a.alpha2;
+  if (a.alpha_tab != nullptr) {
+    int it0;
+    const int n = solve_prologue(a, lblk == 0, &it0);
+    stopped = stopped || n == 0;
+    pair = n == 2;
+    if (n > 0) {
+      alpha1 = a.alpha_tab[it0];
+      alpha2 = pair ? a.alpha_tab[it0 + 1] : 0.0f;
+    }
+  } else if (a.prev_partials != nullptr) {
     const bool st = finalize_previous(a, lblk == 0);
     stopped = stopped || st;
   }
@@ -153,10 +195,21 @@ energy_pair_kernel(const PassArgs<float> a) {
       const int sn = c & 1;
 #pragma unroll
       for (int k = 0; k < NZ; ++k) {
-        adam_step(m[k], gu[k], gv[k], a.omb1, a.omb2, a.alpha, a.eps, a_lo_u[k], a_lo_v[k],
+        adam_step(m[k], gu[k], gv[k], a.omb1, a.omb2, alpha1, a.eps, a_lo_u[k], a_lo_v[k],
                   q_u[k], q_v[k]);
         sm[sn][k][lane] = make_float4(m[k][0], m[k][1], m[k][2], m[k][3]);
         sq[sn][k][lane] = make_float2(q_u[k], q_v[k]);
+      }
+      if (!pair) {
+        // one iteration: U1 and m1 are the pass's output (owned node rows)
+        const uint32_t o1 = node_off(c, own && c >= j0 && c < j1);
+        const uint32_t om1 = o1 == kOOB ? kOOB : 4u * o1;
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          bstore(q_u[k], rUo, o1, k * kstep);
+          bstore(q_v[k], rVo, o1, k * kstep);
+          mstore(m[k], rMo, om1, k * mstep);
+        }
       }
 
       if (own && c >= j0 && c < j1) {
@@ -170,7 +223,7 @@ energy_pair_kernel(const PassArgs<float> a) {
       // only).  In the wave's first two steps (cell rows j0-3 and j0-2) it
       // would only feed node rows below the wave's own, so the wave skips it
       // (a wave-uniform branch).
-      if (c >= j0) {
+      if (pair && c >= j0) {
         const int sp = sn ^ 1;
         T p_u[NZ], p_v[NZ];
 #pragma unroll
@@ -187,7 +240,7 @@ energy_pair_kernel(const PassArgs<float> a) {
               const float4 t = sm[sp][k][lane];
               T mk[4] = {t.x, t.y, t.z, t.w};
               T un_, vn_;
-              adam_step(mk, g_u, g_v, a.omb1, a.omb2, a.alpha2, a.eps, p_u[k], p_v[k], un_, vn_);
+              adam_step(mk, g_u, g_v, a.omb1, a.omb2, alpha2, a.eps, p_u[k], p_v[k], un_, vn_);
               bstore(un_, rUo, on, k * kstep);
               bstore(vn_, rVo, on, k * kstep);
               mstore(mk, rMo, om, k * mstep);
@@ -249,7 +302,7 @@ int pair_impl(int nz, int Ny, int Nx, const float* U, const float* V, const floa
               const IgmAdam* adam2, double* partials, const int32_t* stop,
               const double* prev_partials, int prev_count, const IgmEarlyStop* es_in,
               IgmEarlyStop* es_out, double* fin_out, double* cost_hist, int prev_iter,
-              void* stream) {
+              const float* alpha_tab, int nbitmax, int finalize_only, void* stream) {
   IGM_CHECK_ARG(U && V && coef && hscale && dzeta && phys && partials, "null pointer");
   IGM_CHECK_ARG(Uo && Vo && mom_in && mom_out && adam1 && adam2,
                 "needs Uout, Vout, both moment buffers and both Adam steps");
@@ -272,7 +325,14 @@ int pair_impl(int nz, int Ny, int Nx, const float* U, const float* V, const floa
   a.mom_out = mom_out;
   a.coef = coef; a.hscale = hscale; a.partials = partials; a.stop = stop;
   const PassGeom g = pass_geom(1, Ny, Nx, kPairOut);
-  if (prev_partials) {
+  if (alpha_tab) {
+    IGM_CHECK_ARG(nbitmax >= 1, "nbitmax must be >= 1");
+    IGM_CHECK_ARG(es_in && es_out && fin_out && es_in != es_out,
+                  "a solve pass needs es_in != es_out and out");
+    IGM_CHECK_ARG(prev_partials != partials, "prev_partials must not alias partials");
+    IGM_CHECK_ARG(prev_partials || !finalize_only, "finalize_only needs the previous partials");
+    a.prev_nblk = g.blocks;
+  } else if (prev_partials) {
     IGM_CHECK_ARG(prev_count == 1 || prev_count == 2,
                   "prev_count must be 1 (one-iteration pass) or 2 (two-iteration pass)");
     IGM_CHECK_ARG(es_in && es_out && fin_out && es_in != es_out,
@@ -283,6 +343,7 @@ int pair_impl(int nz, int Ny, int Nx, const float* U, const float* V, const floa
   }
   a.prev_partials = prev_partials; a.es_in = es_in; a.es_out = es_out; a.fin_out = fin_out;
   a.cost_hist = cost_hist; a.prev_iter = prev_iter;
+  a.alpha_tab = alpha_tab; a.nbitmax = nbitmax; a.finalize_only = finalize_only ? 1 : 0;
   a.vel_bytes = (uint32_t)vel_bytes;
   a.coef_bytes = (uint32_t)coef_bytes;
   a.mom_bytes = (uint32_t)mom_bytes;
@@ -329,7 +390,20 @@ int igm_energy_pass2_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, con
                          double* cost_hist, int32_t prev_iter, void* stream) {
   return igm::pair_impl(nz, Ny, Nx, U, V, coef, hscale, dzeta, phys, Uout, Vout, moments_in,
                         moments_out, adam1, adam2, partials, stop_flag, prev_partials, prev_count,
-                        es_in, es_out, out, cost_hist, prev_iter, stream);
+                        es_in, es_out, out, cost_hist, prev_iter, nullptr, 0, 0, stream);
+}
+
+int igm_energy_solve_pass_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, const float* V,
+                              const float* coef, const float* hscale, const double* dzeta,
+                              const IgmPhysics* phys, float* Uout, float* Vout,
+                              const float* moments_in, float* moments_out, const IgmAdam* adam,
+                              const float* alpha_tab, int32_t nbitmax, double* partials,
+                              const double* prev_partials, const IgmEarlyStop* es_in,
+                              IgmEarlyStop* es_out, double* out, double* cost_hist,
+                              int32_t finalize_only, void* stream) {
+  return igm::pair_impl(nz, Ny, Nx, U, V, coef, hscale, dzeta, phys, Uout, Vout, moments_in,
+                        moments_out, adam, adam, partials, nullptr, prev_partials, 0, es_in,
+                        es_out, out, cost_hist, 0, alpha_tab, nbitmax, finalize_only, stream);
 }
 
 }  // extern "C"

@@ -105,6 +105,12 @@ struct PassArgs {
   const double* prev_partials;
   int prev_count;  // solver iterations the previous pass ran (1 or 2)
   int prev_nblk;   // blocks of the previous pass (its partials)
+  // device-driven solve pass (alpha_tab != nullptr): the pass decides from
+  // the EarlyStopping state how many iterations it runs; alpha_tab[i] is the
+  // Adam step of iteration i < nbitmax
+  const float* alpha_tab;
+  int nbitmax;
+  int finalize_only;  // account the previous pass, run nothing
   const IgmEarlyStop* es_in;
   IgmEarlyStop* es_out;
   double* fin_out;
@@ -208,6 +214,17 @@ __device__ bool finalize_previous(const PassArgs<T>& a, bool block0) {
   }
   if (block0 && threadIdx.x == 0) *a.es_out = st;
   return st.stopped != 0;
+}
+
+// Iterations a device-driven solve pass runs when the EarlyStopping state
+// before it is s and its first iteration is f: none once stopped or done;
+// two when the first cannot end the solve (the rule has no best COST yet,
+// or one more non-improving COST leaves wait below patience) and two
+// remain; else one.
+__device__ __forceinline__ int pass_iterations(const IgmEarlyStop& s, int f, int nbitmax) {
+  if (s.stopped || f >= nbitmax) return 0;
+  const bool first_cannot_stop = !s.has_best || (s.wait + 1 < s.patience);
+  return (first_cannot_stop && f + 1 < nbitmax) ? 2 : 1;
 }
 
 // ------------------------------------------------------- level sweep --

@@ -147,6 +147,20 @@ class EnergyOperator:
                  if prev is not None else (None, 0, None, None, None, None, 0)),
                L.stream_ptr())
 
+    def run_solve_pass(self, U, V, Uo, Vo, mom_in, mom_out, adam, alpha_tab, nbitmax, partials,
+                       prev_partials, es_in, es_out, cost_hist, finalize_only=False):
+        """One device-driven solve pass (``igm_energy_solve_pass_f32``): it
+        accounts the previous pass and runs 0, 1 or 2 iterations as the
+        EarlyStopping state allows.  ``alpha_tab``: float32 device tensor of
+        the per-iteration Adam steps; ``es_in``, ``es_out``: device addresses
+        of two IgmEarlyStop states."""
+        assert self.pair_supported and U.dim() == 3 and U.is_contiguous() and V.is_contiguous()
+        L.call("igm_energy_solve_pass_f32", self.nz, self.Ny, self.Nx, L.ptr(U), L.ptr(V),
+               L.ptr(self.coef), L.ptr(self.hscale), C.addressof(self.dzeta), C.addressof(self.phys),
+               L.ptr(Uo), L.ptr(Vo), L.ptr(mom_in), L.ptr(mom_out), C.addressof(adam),
+               L.ptr(alpha_tab), int(nbitmax), L.ptr(partials), L.ptr(prev_partials), es_in, es_out,
+               L.ptr(self.out), L.ptr(cost_hist), 1 if finalize_only else 0, L.stream_ptr())
+
     def finalize_pair(self, es=None, cost_hist=None, it=0, partials=None, out=None):
         """``finalize`` of a ``run_pass2`` call: iterations it and it+1."""
         out = self.out if out is None else out

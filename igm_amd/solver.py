@@ -32,15 +32,23 @@ class LegacyAdamState:
         self.lr, self.beta1, self.beta2, self.epsilon = lr, beta1, beta2, epsilon
         self.iterations = 0
 
-    def alpha(self, t, dtype=torch.float32):
-        """lr * sqrt(1 - beta2^t) / (1 - beta1^t) at step t (1-based), in the
-        working precision (float32 for the float32 solver, as TF computes it)."""
+    def alphas(self, t_first, lrs, dtype=torch.float32):
+        """lr_i * sqrt(1 - beta2^t) / (1 - beta1^t) for the steps t = t_first,
+        t_first + 1, ... (1-based) with step sizes lrs, in the working
+        precision (float32 for the float32 solver, as TF computes it).
+        Returns a numpy array of that precision."""
         if dtype == torch.float32:
             f = np.float32
-            b1p = f(np.power(f(self.beta1), f(t)))
-            b2p = f(np.power(f(self.beta2), f(t)))
-            return float(f(self.lr) * np.sqrt(f(1) - b2p) / (f(1) - b1p))
-        return self.lr * np.sqrt(1 - self.beta2 ** t) / (1 - self.beta1 ** t)
+            t = np.arange(t_first, t_first + len(lrs), dtype=np.float64).astype(f)
+            b1p = np.power(f(self.beta1), t)
+            b2p = np.power(f(self.beta2), t)
+            return np.asarray(lrs, dtype=np.float64).astype(f) * np.sqrt(f(1) - b2p) / (f(1) - b1p)
+        t = np.arange(t_first, t_first + len(lrs), dtype=np.float64)
+        return np.asarray(lrs, dtype=np.float64) * np.sqrt(1 - self.beta2 ** t) / (1 - self.beta1 ** t)
+
+    def alpha(self, t, dtype=torch.float32):
+        """The step alpha at step t (1-based) with the current lr (see ``alphas``)."""
+        return float(self.alphas(t, [self.lr], dtype)[0])
 
 
 class SolveResult:
@@ -57,10 +65,11 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     applied.  ``lr_schedule(i)`` (optional) overrides the learning rate of
     iteration i (the reference uses a constant step_size).
 
-    Without early stopping there is no decision between iterations, and the
-    float32 solver runs two iterations per kernel launch (``run_pass2``,
-    which gives the same result as two one-iteration passes); ``pairs=False``
-    keeps one iteration per launch.
+    The float32 solver runs device-driven passes (``run_solve_pass``): each
+    runs two iterations whenever the first of them cannot trigger the
+    EarlyStopping rule -- always, without early stopping -- and gives the
+    same result as one iteration per pass; ``pairs=False`` keeps one
+    iteration per launch.
     """
     dt = op.dtype
     assert U0.dtype == dt and U0.shape == V0.shape and U0.dim() == 3
@@ -86,57 +95,56 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     adam = L.IgmAdam(alpha=0.0, beta1=optimizer.beta1, beta2=optimizer.beta2,
                      epsilon=optimizer.epsilon)
     t0 = optimizer.iterations
-    lr0 = optimizer.lr
+    # the Adam step of every iteration, as the legacy optimizer computes it
+    lrs = [lr_schedule(i) if lr_schedule is not None else optimizer.lr for i in range(nbitmax)]
+    alphas = optimizer.alphas(t0 + 1, lrs, dt)
 
-    def alpha_of(i):
-        if lr_schedule is not None:
-            optimizer.lr = lr_schedule(i)
-        return optimizer.alpha(t0 + i + 1, dt)
-
-    if pairs and not early_stopping and op.pair_supported and nbitmax >= 2:
-        # Pass k covers one iteration (only the first pass, when nbitmax is
-        # odd) or two; it finalizes the previous pass's iterations itself,
-        # with the same double-buffered states and partials as below.  The
-        # moments ping-pong between two buffers.
-        M = [mom, torch.empty_like(mom)]
-        adam2 = L.IgmAdam(alpha=0.0, beta1=optimizer.beta1, beta2=optimizer.beta2,
-                          epsilon=optimizer.epsilon)
-        i = k = src = cur = 0
-        n_prev = 0
-        while i < nbitmax:
-            if k == 0 and nbitmax % 2 == 1:
-                adam.alpha = alpha_of(i)
-                op.run_pass(2, Ub[src], Vb[src], Uo=Ub[1 - src], Vo=Vb[1 - src], moments=M[cur],
-                            adam=adam, partials=P[k % 2])
-                n_it = 1
-            else:
-                adam.alpha = alpha_of(i)
-                adam2.alpha = alpha_of(i + 1)
-                prev = None if k == 0 else (P[(k - 1) % 2], n_prev, E[k % 2], E[(k - 1) % 2], hist,
-                                            i - n_prev)
-                op.run_pass2(Ub[src], Vb[src], Ub[1 - src], Vb[1 - src], M[cur], M[1 - cur], adam,
-                             adam2, P[k % 2], prev=prev)
-                cur = 1 - cur
-                n_it = 2
-            src = 1 - src
-            i += n_it
-            n_prev = n_it
-            k += 1
-        last = k % 2
-        if n_prev == 1:
-            op.finalize(es=E[last], cost_hist=hist, it=i - 1, partials=P[(k - 1) % 2])
-        else:
-            op.finalize_pair(es=E[last], cost_hist=hist, it=i - 2, partials=P[(k - 1) % 2])
-        optimizer.lr = lr0
+    def result(last):
         raw = bytes(es_dev.cpu().numpy().tobytes())
-        st = L.IgmEarlyStop.from_buffer_copy(raw[last * es_size:(last + 1) * es_size])
+        return L.IgmEarlyStop.from_buffer_copy(raw[last * es_size:(last + 1) * es_size])
+
+    if pairs and op.pair_supported and nbitmax >= 1:
+        # Device-driven passes: launch k reads buffer k % 2 of U, V and the
+        # moments and writes buffer (k + 1) % 2; it accounts the previous
+        # launch's iterations itself (reading state E[k % 2], writing
+        # E[(k + 1) % 2]) and runs two iterations when the first cannot end
+        # the solve, else one, else none.  Without early stopping that is
+        # pairs throughout (the last one single when nbitmax is odd); with
+        # it, the host reads the state every check_every launches and stops
+        # launching once the solve has ended.  A last launch accounts the
+        # final pass; the field is then in buffer passes % 2.
+        M = [mom, torch.empty_like(mom)]
+        atab = torch.from_numpy(np.ascontiguousarray(alphas, dtype=np.float32)).to(dev)
+
+        def launch(k, finalize_only=False):
+            op.run_solve_pass(Ub[k % 2], Vb[k % 2], Ub[(k + 1) % 2], Vb[(k + 1) % 2], M[k % 2],
+                              M[(k + 1) % 2], adam, atab, nbitmax, P[k % 2],
+                              P[(k - 1) % 2] if k > 0 else None, E[k % 2], E[(k + 1) % 2], hist,
+                              finalize_only)
+
+        k = 0
+        if not early_stopping:
+            for k in range((nbitmax + 1) // 2):
+                launch(k)
+            k = (nbitmax + 1) // 2
+        else:
+            while k < nbitmax:
+                launch(k)
+                k += 1
+                if k >= 2 and k % check_every == 0:
+                    st = result(k % 2)  # the state before launch k-1 ran
+                    if st.stopped or st.iters_done >= nbitmax:
+                        break
+        launch(k, finalize_only=True)
+        st = result((k + 1) % 2)
         n = int(st.iters_done)
         optimizer.iterations = t0 + n
-        return SolveResult(Ub[src], Vb[src], hist[:n].cpu().numpy(), n, bool(st.stopped))
+        r = int(st.passes) % 2
+        return SolveResult(Ub[r], Vb[r], hist[:n].cpu().numpy(), n, bool(st.stopped))
 
     n_launched = 0
     for i in range(nbitmax):
-        adam.alpha = alpha_of(i)
+        adam.alpha = float(alphas[i])
         src, dst = i % 2, (i + 1) % 2
         prev = None if i == 0 else (P[(i - 1) % 2], E[i % 2], E[(i - 1) % 2], hist, i - 1)
         op.run_pass(2, Ub[src], Vb[src], Uo=Ub[dst], Vo=Vb[dst], moments=mom, adam=adam,
@@ -151,9 +159,7 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
         op.finalize(es=E[last], cost_hist=hist, it=n_launched - 1, partials=P[(n_launched - 1) % 2])
     else:
         last = 0
-    optimizer.lr = lr0
-    raw = bytes(es_dev.cpu().numpy().tobytes())
-    st = L.IgmEarlyStop.from_buffer_copy(raw[last * es_size:(last + 1) * es_size])
+    st = result(last)
     n = int(st.iters_done)
     optimizer.iterations = t0 + n
     res = n % 2

@@ -61,6 +61,7 @@ typedef struct IgmEarlyStop {
   int32_t stopped;     /* set to 1 when should_stop() returned True */
   int32_t stop_iter;   /* iteration index at which it stopped */
   int32_t iters_done;  /* number of iterations whose update was applied */
+  int32_t passes;      /* igm_energy_solve_pass_f32 launches that applied an update */
 } IgmEarlyStop;
 
 const char* igm_last_error(void);
@@ -165,6 +166,31 @@ int igm_energy_pass2_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, con
                          const double* prev_partials, int32_t prev_count,
                          const IgmEarlyStop* es_in, IgmEarlyStop* es_out, double* out,
                          double* cost_hist, int32_t prev_iter, void* stream);
+
+/* One pass of a solve whose iteration count is decided on the device
+ * (solve/solve.py:45-84 with the EarlyStopping check of solve.py:112): the
+ * pass first accounts the previous pass's iterations (their COSTs into
+ * cost_hist, the EarlyStopping rule applied to a copy of *es_in, the new
+ * state to *es_out), then runs two iterations when the first of them cannot
+ * end the solve (the rule has no best COST yet, or one more non-improving
+ * COST leaves wait below patience), one when it can, none once stopped or
+ * when iters_done has reached nbitmax.  Results equal one-iteration passes.
+ * Launch k (from 0) reads U, V and moments from buffer k % 2 and writes
+ * buffer (k + 1) % 2 of each pair, partials to a buffer of its own (two in
+ * turn), es_in = state k % 2, es_out = state (k + 1) % 2; launch 0 gets no
+ * prev_partials.  A last launch with finalize_only = 1 accounts the final
+ * pass; the result is then in buffer passes % 2 (IgmEarlyStop.passes).
+ * alpha_tab: device array of the Adam step alpha of iterations 0..nbitmax-1;
+ * adam: beta1, beta2 and epsilon.  Float32, planar (Nz, Ny, Nx), one field;
+ * nz: 2, 3, 4, 5, 6, 8 or 10. */
+int igm_energy_solve_pass_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, const float* V,
+                              const float* coef, const float* hscale, const double* dzeta,
+                              const IgmPhysics* phys, float* Uout, float* Vout,
+                              const float* moments_in, float* moments_out, const IgmAdam* adam,
+                              const float* alpha_tab, int32_t nbitmax, double* partials,
+                              const double* prev_partials, const IgmEarlyStop* es_in,
+                              IgmEarlyStop* es_out, double* out, double* cost_hist,
+                              int32_t finalize_only, void* stream);
 
 /* igm_energy_finalize for the partials of an igm_energy_pass2_f32 call:
  * COST and EarlyStopping (solve.py:57-59, 112) of iterations iter and

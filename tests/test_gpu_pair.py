@@ -137,3 +137,35 @@ def test_solver_pairs_equal_single_iterations(dev, nit):
     c1, c2 = np.asarray(r1.costs), np.asarray(r2.costs)
     assert c1.shape == c2.shape == (nit,)
     assert np.all(np.abs(c2 - c1) <= 1e-12 * np.abs(c1)), (c1, c2)
+
+
+@pytest.mark.parametrize("patience,delta,nit,warm", [(1, 2e-2, 60, 0), (2, 5e-3, 80, 0),
+                                                      (3, 2e-3, 100, 0), (10, 2e-4, 100, 0),
+                                                      (10, 2e-4, 7, 0), (3, 2e-2, 100, 0),
+                                                      (10, 5e-2, 100, 0), (10, 2e-4, 100, 80),
+                                                      (4, 1e-3, 100, 80), (3, 5e-4, 100, 80),
+                                                      (5, 3e-4, 100, 80), (2, 3e-4, 100, 40)])
+def test_solver_with_early_stopping_on_pairs(dev, patience, delta, nit, warm):
+    """With early stopping the device decides, pass by pass, whether two
+    iterations are safe (the first cannot end the solve) or one: the result
+    is the one-iteration solver's -- same stop, field, COSTs and counter."""
+    print(f"case solver pairs + early stopping patience={patience} delta={delta} nit={nit} warm={warm}",
+          flush=True)
+    g = small_glacier(90, 140, seed=5)
+    op = gpu_op(g, 10, dev)
+    z = torch.zeros((10, 90, 140), device=dev)
+    U0, V0 = z, z.clone()
+    if warm:  # a warm start, as in a transient run's update()
+        r0 = solve(op, z, z.clone(), warm, LegacyAdamState(lr=1.0), early_stopping=False, pairs=False)
+        U0, V0 = r0.U, r0.V
+    o1, o2 = LegacyAdamState(lr=1.0), LegacyAdamState(lr=1.0)
+    kw = dict(early_stopping=True, relative_min_delta=delta, patience=patience, check_every=4)
+    r1 = solve(op, U0, V0, nit, o1, pairs=False, **kw)
+    r2 = solve(op, U0, V0, nit, o2, pairs=True, **kw)
+    print(f"  single: {r1.iterations} its stopped {r1.stopped}; pairs: {r2.iterations} its stopped {r2.stopped}")
+    assert (r2.iterations, r2.stopped) == (r1.iterations, r1.stopped)
+    assert o1.iterations == o2.iterations == r1.iterations
+    assert first_mismatch(r2.U, r1.U) is None and first_mismatch(r2.V, r1.V) is None
+    c1, c2 = np.asarray(r1.costs), np.asarray(r2.costs)
+    assert c1.shape == c2.shape
+    assert np.all(np.abs(c2 - c1) <= 1e-12 * np.abs(c1)), (c1, c2)
