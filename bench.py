@@ -366,8 +366,12 @@ def main():
     log(f"[bench] solver: {n1} its (lr0 {lr1}) to err {solver_err:.4f} (<= 1%: {solver_reached}) in "
         f"{solver_ms:.1f} ms ({it_ms * 1e3:.1f} us/it) -> {solver_fps:.3f} fields/s; reference settings {curve}")
     # reference early-stopping behaviour (nbitmax large)
-    res_es = solve(op32, z, z.clone(), 20000, LegacyAdamState(lr=1.0), early_stopping=True)
+    with Timer() as tm:
+        res_es = solve(op32, z, z.clone(), 20000, LegacyAdamState(lr=1.0), early_stopping=True)
+    es_ms = tm.ms
     es_err = rel_err(res_es.U, res_es.V, Ur, Vr, mask)
+    log(f"[bench] reference settings with EarlyStopping: stops after {res_es.iterations} its "
+        f"({es_ms:.1f} ms, {es_ms * 1e3 / max(1, res_es.iterations):.1f} us/it), rel err {es_err:.4f}")
 
     # ------------------------------------------------------- rooflines --
     kt = measure_kernels(state.iceflow_model, op32, Ur.float().contiguous(), Vr.float().contiguous(), f32,
@@ -484,7 +488,8 @@ def main():
         "solver_reference_settings": ref_settings,
         "solver_us_per_iter": round(it_ms * 1e3, 2),
         "emulator_vs_solver_speedup": round(emu_fps / solver_fps, 1),
-        "solver_early_stopping": {"iterations": int(res_es.iterations), "rel_err": round(es_err, 5)},
+        "solver_early_stopping": {"iterations": int(res_es.iterations), "rel_err": round(es_err, 5),
+                                  "ms": round(es_ms, 2)},
         "roofline_fused_single": {"bound": "hbm", "achieved": round(fused_gbs, 1), "peak": PEAK_HBM_GBS,
                                   "unit": "GB/s", "frac": round(fused_gbs / PEAK_HBM_GBS, 4),
                                   "kernel": "energy_pass_kernel<float, 10, 2, 0> (one fused iteration)",
