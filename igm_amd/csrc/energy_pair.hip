@@ -78,7 +78,8 @@ energy_pair_kernel(const PassArgs<float> a) {
   double e1_sh = 0.0, e1_sl = 0.0, e1_gr = 0.0;  // energy at U0 (first iteration)
   double e2_sh = 0.0, e2_sl = 0.0, e2_gr = 0.0;  // energy at U1 (second iteration)
   // per wave: two slots of one node row's m1 (16 B) and U1 (8 B) per level
-  // and lane -- 120 KB per block at Nz 10, within gfx950's 160 KB of LDS
+  // and lane -- 120 KB per block at Nz 10, 144 KB at Nz 12, within gfx950's
+  // 160 KB of LDS
   __shared__ float4 lds_m[kWavesPerBlock][2][NZ][kWave];
   __shared__ float2 lds_q[kWavesPerBlock][2][NZ][kWave];
   bool stopped = (a.stop != nullptr) && (*a.stop != 0);
@@ -367,9 +368,10 @@ int pair_impl(int nz, int Ny, int Nx, const float* U, const float* V, const floa
     IGM_PAIR_CASE(6)
     IGM_PAIR_CASE(8)
     IGM_PAIR_CASE(10)
+    IGM_PAIR_CASE(12)
 #undef IGM_PAIR_CASE
     default:
-      set_error("igm_energy_pass2_f32: nz not compiled (supported: 2,3,4,5,6,8,10)");
+      set_error("igm_energy_pass2_f32: nz not compiled (supported: 2,3,4,5,6,8,10,12)");
       return -1;
   }
   IGM_CHECK_LAUNCH();

@@ -122,7 +122,7 @@ class EnergyOperator:
                L.stream_ptr())
 
     # Nz with a two-iteration pass (igm_energy_pass2_f32)
-    PAIR_NZ = (2, 3, 4, 5, 6, 8, 10)
+    PAIR_NZ = (2, 3, 4, 5, 6, 8, 10, 12)
 
     @property
     def pair_supported(self):

@@ -157,7 +157,7 @@ int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int3
  * igm_energy_pass_f32; prev_count = 1 when the previous pass was an
  * igm_energy_pass_f32 call, 2 when it was an igm_energy_pass2_f32 call (its
  * two iterations prev_iter and prev_iter+1 are accounted in order).
- * nz: 2, 3, 4, 5, 6, 8 or 10. */
+ * nz: 2, 3, 4, 5, 6, 8, 10 or 12. */
 int igm_energy_pass2_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, const float* V,
                          const float* coef, const float* hscale, const double* dzeta,
                          const IgmPhysics* phys, float* Uout, float* Vout,
@@ -182,7 +182,7 @@ int igm_energy_pass2_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, con
  * pass; the result is then in buffer passes % 2 (IgmEarlyStop.passes).
  * alpha_tab: device array of the Adam step alpha of iterations 0..nbitmax-1;
  * adam: beta1, beta2 and epsilon.  Float32, planar (Nz, Ny, Nx), one field;
- * nz: 2, 3, 4, 5, 6, 8 or 10. */
+ * nz: 2, 3, 4, 5, 6, 8, 10 or 12. */
 int igm_energy_solve_pass_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, const float* V,
                               const float* coef, const float* hscale, const double* dzeta,
                               const IgmPhysics* phys, float* Uout, float* Vout,

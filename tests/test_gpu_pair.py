@@ -58,7 +58,7 @@ def first_mismatch(a, b):
 # (ny, nx, nz): ragged widths (not multiples of 60 or 62), few rows per
 # wave, a single strip, tiny grids
 CASES = [(100, 100, 10), (37, 131, 5), (257, 61, 2), (8, 300, 3), (5, 7, 4), (130, 181, 8),
-         (64, 120, 6), (2, 2, 10), (3, 62, 10), (61, 60, 10)]
+         (64, 120, 6), (2, 2, 10), (3, 62, 10), (61, 60, 10), (50, 70, 12)]
 
 
 @pytest.mark.parametrize("ny,nx,nz", CASES)
@@ -115,8 +115,8 @@ def test_pair_errors(dev):
     with pytest.raises(L.IgmError, match="share"):
         op.run_pass2(U0, U0.clone(), torch.empty_like(U0), torch.empty_like(U0), m, m.clone(), a,
                      b, op.partials)
-    op12 = gpu_op(g, 12, dev)
-    assert not op12.pair_supported
+    op16 = gpu_op(g, 16, dev)
+    assert not op16.pair_supported
 
 
 @pytest.mark.parametrize("nit", [1, 2, 7, 20, 33])
