@@ -49,3 +49,45 @@ def velocity_outputs(U, V, thk, vert_weight, nz, write_uv=True):
            L.stream_ptr())
     out["U"], out["V"] = Uo, Vo
     return out
+
+
+def misfits(thk, U_ref, V_ref, U, V, vert_weight, vert_spacing):
+    """The reference's accuracy measures of a velocity field (U, V) against
+    (U_ref, V_ref), all planar (Nz, Ny, Nx) on any device, in float64:
+
+    * ``rel_l1``, ``rel_l2``: pretraining.py:502-535 (_computemisfitall) --
+      the thickness-weighted L1 and L2 norms of the layer-midpoint velocity
+      difference over the ice column, relative to those of the reference
+      field; the layer thicknesses are thk times the steps of the stretched
+      vertical coordinate (vert_disc.py:10-13);
+    * ``velbar_l1``, ``velbar_rms``: diagnostic.py:74-84 (computemisfit) --
+      the mean absolute and root-mean-square difference of the
+      depth-averaged velocity over cells with thk > 1.
+
+    A measurement for reports (float64 torch reductions), not a hot path.
+    """
+    nz = U.shape[0]
+    f64 = torch.float64
+    thk, U_ref, V_ref, U, V = (t.to(f64) for t in (thk, U_ref, V_ref, U, V))
+    zeta = torch.arange(nz, dtype=f64, device=U.device) / (nz - 1)
+    lev = (zeta / vert_spacing) * (1.0 + (vert_spacing - 1.0) * zeta)
+    dz = (lev[1:] - lev[:-1])[:, None, None] * thk[None]
+
+    def mid(a):
+        return 0.5 * (a[1:] + a[:-1])
+
+    du, dv = mid(U_ref) - mid(U), mid(V_ref) - mid(V)
+    ur, vr = mid(U_ref), mid(V_ref)
+
+    def normp(a, b, p):
+        return float(torch.sum(dz * (a.abs() ** p + b.abs() ** p)) ** (1.0 / p))
+
+    w = torch.as_tensor(vert_weight).to(device=U.device, dtype=f64).reshape(-1)[:, None, None]
+    dub, dvb = torch.sum(w * (U_ref - U), 0), torch.sum(w * (V_ref - V), 0)
+    m = (thk > 1).to(f64)
+    n = 2.0 * float(torch.sum(m))
+    l1 = (float(torch.sum(dub.abs() * m) + torch.sum(dvb.abs() * m)) / n) if n else 0.0
+    l2 = (float(torch.sum(dub ** 2 * m) + torch.sum(dvb ** 2 * m)) / n) if n else 0.0
+    return {"rel_l1": normp(du, dv, 1.0) / normp(ur, vr, 1.0),
+            "rel_l2": normp(du, dv, 2.0) / normp(ur, vr, 2.0),
+            "velbar_l1": l1, "velbar_rms": float(np.sqrt(l2))}

@@ -83,3 +83,31 @@ def rel_l2(a, b, mask=None):
         a = a * m
         b = b * m
     return float(torch.linalg.vector_norm(a - b) / torch.clamp(torch.linalg.vector_norm(b), min=1e-300))
+
+
+def computemisfitall(thk, Ut, Vt, Up, Vp, vert_weight, vert_spacing):
+    """pretraining.py:502-535: (rel L1, rel L2) of the layer-midpoint velocity
+    difference weighted by the layer thickness thk*(lev_{l+1}-lev_l), with the
+    stretched levels of vert_disc.py:10-13, and computemisfit's (L1, RMS) of
+    the velbar difference.  (The reference returns the velbar L1 twice in
+    place of (L1, L2); the restatement returns both.)  Test infrastructure."""
+    Nz = Ut.shape[0]
+    thk = torch.as_tensor(thk, dtype=torch.float64)
+    zeta = np.arange(Nz) / (Nz - 1)
+    temp = (zeta / vert_spacing) * (1.0 + (vert_spacing - 1.0) * zeta)
+    temp = temp[1:] - temp[:-1]
+    dz = torch.stack([thk * float(z) for z in temp])
+    Ut, Vt, Up, Vp = (torch.as_tensor(a, dtype=torch.float64) for a in (Ut, Vt, Up, Vp))
+    l1bar, l2bar = computemisfit(thk, Ut - Up, Vt - Vp, vert_weight)
+
+    def half(a):
+        return 0.5 * (a[1:, :, :] + a[:-1, :, :])
+
+    ut, up, vt, vp = half(Ut), half(Up), half(Vt), half(Vp)
+
+    def normp(u, v, p):
+        temp = torch.sum(dz * (torch.abs(u) ** p + torch.abs(v) ** p), dim=0)
+        return float(torch.sum(temp) ** (1.0 / p))
+
+    return (normp(ut - up, vt - vp, 1.0) / normp(ut, vt, 1.0),
+            normp(ut - up, vt - vp, 2.0) / normp(ut, vt, 2.0), l1bar, l2bar)
