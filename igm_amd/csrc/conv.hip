@@ -22,7 +22,8 @@
 //                       into LDS as [channel][pixel] (three x-shifted copies
 //                       of the activation rows keep every fragment read
 //                       16-B aligned); per-block partial sums, then a
-//                       deterministic reduction (wgrad_reduce_kernel).
+//                       deterministic two-stage reduction
+//                       (partials_group_sum_kernel, wgrad_reduce_kernel).
 //  head_bwd_kernel      dY -> dA16 * LeakyReLU'(A16) (fp32 VALU) and a padded
 //                       bf16 copy of dY for the head's wgrad.
 //  conv_in_wgrad_kernel layer-1 weight gradient, fp32 VALU.
@@ -597,20 +598,44 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
 // db[co] = sum_blk P[blk][TAPS*1024 + co].  A block reduces 64 consecutive
 // partial entries: thread (g, j) sums blocks g, g+4, ... of entry j, then the
 // four group sums are added in a fixed order (deterministic).
+// Deterministic reduction of per-block partials P[nblk][stride], in two
+// stages.  Stage 1 (partials_group_sum_kernel): group g of R rows,
+// rows g*R .. min(nblk, (g+1)*R) - 1, is summed in row order into its first
+// row, in place; each (group, column) belongs to one thread, so the write
+// races with nothing, and the loads are issued 8 at a time.  Stage 2 sums
+// the group rows in group order.  The fixed order makes the result
+// independent of scheduling.
 __global__ void __launch_bounds__(256)
-wgrad_reduce_kernel(const float* __restrict__ P, int nblk, int taps, int cout, int accumulate,
-                    float* __restrict__ dW, float* __restrict__ db) {
-  __shared__ float part[4][64];
-  const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int stride = taps * kC * kC + kC;
-  const int src = blockIdx.x * 64 + j;  // partial-entry index
+partials_group_sum_kernel(float* __restrict__ P, int nblk, int stride, int ncols, int R) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const int r0 = blockIdx.y * R;
+  if (c >= ncols || r0 >= nblk) return;
+  const int n = min(nblk, r0 + R) - r0;
+  float* p = P + (int64_t)r0 * stride + c;
   float s = 0.f;
-  if (src < stride)
-    for (int k = g; k < nblk; k += 4) s += P[(int64_t)k * stride + src];
-  part[g][j] = s;
-  __syncthreads();
-  if (g != 0 || src >= stride) return;
-  const float t = ((part[0][j] + part[1][j]) + part[2][j]) + part[3][j];
+  int r = 0;
+  for (; r + 8 <= n; r += 8) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = p[(int64_t)(r + i) * stride];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s += v[i];
+  }
+  for (; r < n; ++r) s += p[(int64_t)r * stride];
+  p[0] = s;
+}
+
+// Stage 2 for a 3x3 or 1x1 layer: the group sums of entry src = ((tap * 32
+// + co) * 32 + ci) (then the 32 bias entries) scattered to dW (HWIO, cout
+// output channels) and db.
+__global__ void __launch_bounds__(256)
+wgrad_reduce_kernel(const float* __restrict__ P, int ngroups, int R, int taps, int cout,
+                    int accumulate, float* __restrict__ dW, float* __restrict__ db) {
+  const int stride = taps * kC * kC + kC;
+  const int src = blockIdx.x * 256 + threadIdx.x;
+  if (src >= stride) return;
+  float t = 0.f;
+  for (int g = 0; g < ngroups; ++g) t += P[(int64_t)g * R * stride + src];
   if (src < taps * kC * kC) {
     const int ci = src % kC, co = (src / kC) % kC, tap = src / (kC * kC);
     if (co < cout) {
@@ -623,93 +648,125 @@ wgrad_reduce_kernel(const float* __restrict__ P, int nblk, int taps, int cout, i
   }
 }
 
+// Stage 1 + 2 on the host side: G groups of R rows, G <= kMaxGroups.
+constexpr int kMaxGroups = 16;
+inline void group_rows(int nblk, int* R, int* G) {
+  *G = nblk < kMaxGroups ? nblk : kMaxGroups;
+  *R = (nblk + *G - 1) / *G;
+  *G = (nblk + *R - 1) / *R;
+}
+
 // ------------------------------------------------------ layer-1 wgrad --
 // dW1[r][s][ci][co] (HWIO, fp32) = sum_p X[p + delta][ci] * dZ1[p][co]; db1.
-// Tiles of 1 row x 256 pixels; thread t: co = t % 32, owns pairs
-// (tap*5 + ci) = g, g+8, ... (g = t / 32).  partials: [grid][1440 + 32].
-constexpr int kInTile = 256;
+// Persistent blocks over tiles of 4 rows x 64 pixels; wave w takes row w of
+// a tile.  Lane l accumulates a 6 x 4 register tile of the output: the
+// (tap, ci) pairs m = 6 (l / 8) .. +5 (m = tap * 5 + ci; 48 slots, the last
+// three empty) by the output channels co = 4 (l % 8) .. +3.  Per pixel, six
+// LDS reads of X and one 16-B read of dZ feed 24 FMAs.  At the end the four
+// waves' tiles are summed in wave order into partials[blk][1440 + 32]
+// (P[m * 32 + co], then the 32 bias sums).
+constexpr int kWgTileW = 64, kWgTileH = 4;
+constexpr int kWgOut = 9 * kCin * kC + kC;  // 1472 partial entries per block
 __global__ void __launch_bounds__(256)
 conv_in_wgrad_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
                      const float* __restrict__ f2, const float* __restrict__ f3,
                      const float* __restrict__ f4, int64_t fb, int64_t fr, int64_t fc,
                      const int32_t* __restrict__ ext, int B, int H, int W,
                      const bf16* __restrict__ dz, float* __restrict__ partials) {
-  __shared__ float xs[3][kInTile + 2][kCin];
-  __shared__ float dzs[kInTile][kC + 1];
-  const int tid = threadIdx.x;
-  const int co = tid & 31, g = tid >> 5;
+  __shared__ float xs[kWgTileH + 2][kWgTileW + 2][kCin];
+  __shared__ float4 dzs[kWgTileH][kWgTileW][kC / 4];
+  __shared__ float red[4][kWgOut];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int mb = lane >> 3, nb = lane & 7;
   const float* fs[kCin] = {f0, f1, f2, f3, f4};
-  float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  float dbacc = 0.f;
-  const int ntx = (W + kInTile - 1) / kInTile;
-  const int64_t ntiles = (int64_t)B * H * ntx;
+  float acc[6][4], dbacc[4] = {0.f, 0.f, 0.f, 0.f};
+  int xoff[6];  // LDS offset of the lane's (tap, ci) pairs from xs[w][px][0]
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const int m = min(6 * mb + j, 9 * kCin - 1), tap = m / kCin, ci = m % kCin;
+    xoff[j] = ((tap / 3) * (kWgTileW + 2) + tap % 3) * kCin + ci;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[j][q] = 0.f;
+  }
+  const int ntx = (W + kWgTileW - 1) / kWgTileW, nty = (H + kWgTileH - 1) / kWgTileH;
+  const int64_t ntiles = (int64_t)B * nty * ntx;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int b = (int)(tile / ((int64_t)H * ntx));
-    const int y = (int)((tile / ntx) % H);
-    const int x0 = (int)(tile % ntx) * kInTile;
+    const int b = (int)(tile / ((int64_t)nty * ntx));
+    const int y0 = (int)((tile / ntx) % nty) * kWgTileH;
+    const int x0 = (int)(tile % ntx) * kWgTileW;
     int ny, nx;
     extent_of(ext, b, H, W, ny, nx);
-    if (y >= ny || x0 >= nx) continue;
+    if (y0 >= ny || x0 >= nx) continue;
     __syncthreads();
-    for (int c = tid; c < 3 * (kInTile + 2) * kCin; c += 256) {
-      const int ci = c % kCin, t = (c / kCin) % (kInTile + 2), row = c / (kCin * (kInTile + 2));
-      const int yy = y - 1 + row, xx = x0 - 1 + t;
+    // X with a one-pixel halo, zero outside the entry's extent
+    for (int c = tid; c < (kWgTileH + 2) * (kWgTileW + 2) * kCin; c += 256) {
+      const int ci = c % kCin, t = (c / kCin) % (kWgTileW + 2), row = c / (kCin * (kWgTileW + 2));
+      const int yy = y0 - 1 + row, xx = x0 - 1 + t;
       float v = 0.f;
       if (yy >= 0 && yy < ny && xx >= 0 && xx < nx)
         v = fs[ci][(int64_t)b * fb + (int64_t)yy * fr + (int64_t)xx * fc];
       xs[row][t][ci] = v;
     }
-    for (int c = tid; c < kInTile * 4; c += 256) {
-      const int px = c >> 2, chunk = c & 3;
-      const int x = x0 + px;
+    // dZ1 (bf16, 32 channels), zero outside the extent
+    for (int c = tid; c < kWgTileH * kWgTileW * 4; c += 256) {
+      const int chunk = c & 3, px = (c >> 2) % kWgTileW, row = (c >> 2) / kWgTileW;
+      const int y = y0 + row, x = x0 + px;
       bf16x8 v;
-      if (x < nx) v = *reinterpret_cast<const bf16x8*>(dz + (((int64_t)b * H + y) * W + x) * kC + chunk * 8);
-      else {
+      if (y < ny && x < nx) {
+        v = *reinterpret_cast<const bf16x8*>(dz + (((int64_t)b * H + y) * W + x) * kC + chunk * 8);
+      } else {
 #pragma unroll
         for (int e = 0; e < 8; ++e) v[e] = f2bf(0.f);
       }
-#pragma unroll
-      for (int e = 0; e < 8; ++e) dzs[px][chunk * 8 + e] = bf2f(v[e]);
+      dzs[row][px][2 * chunk] = make_float4(bf2f(v[0]), bf2f(v[1]), bf2f(v[2]), bf2f(v[3]));
+      dzs[row][px][2 * chunk + 1] = make_float4(bf2f(v[4]), bf2f(v[5]), bf2f(v[6]), bf2f(v[7]));
     }
     __syncthreads();
-    for (int px = 0; px < kInTile; ++px) {
-      const float d = dzs[px][co];
-      dbacc += d;
+    const float* xw = &xs[w][0][0];
+    for (int px = 0; px < kWgTileW; ++px) {
+      const float4 d = dzs[w][px][nb];
+      dbacc[0] += d.x;
+      dbacc[1] += d.y;
+      dbacc[2] += d.z;
+      dbacc[3] += d.w;
 #pragma unroll
       for (int j = 0; j < 6; ++j) {
-        const int pair = g + 8 * j;
-        if (pair < 9 * kCin) {
-          const int tap = pair / kCin, ci = pair % kCin;
-          acc[j] += xs[tap / 3][px + tap % 3][ci] * d;
-        }
+        const float xv = xw[px * kCin + xoff[j]];
+        acc[j][0] += xv * d.x;
+        acc[j][1] += xv * d.y;
+        acc[j][2] += xv * d.z;
+        acc[j][3] += xv * d.w;
       }
     }
   }
-  float* P = partials + (int64_t)blockIdx.x * (9 * kCin * kC + kC);
+  // the four waves' tiles, summed in wave order
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    const int pair = g + 8 * j;
-    if (pair < 9 * kCin) P[pair * kC + co] = acc[j];
+    const int m = 6 * mb + j;
+    if (m < 9 * kCin) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) red[w][m * kC + 4 * nb + q] = acc[j][q];
+    }
   }
-  if (g == 0) P[9 * kCin * kC + co] = dbacc;
+  if (mb == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) red[w][9 * kCin * kC + 4 * nb + q] = dbacc[q];
+  }
+  __syncthreads();
+  float* P = partials + (int64_t)blockIdx.x * kWgOut;
+  for (int i = tid; i < kWgOut; i += 256) P[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
 }
 
-// out[i] = sum_blk P[blk][i], i < count (same 2-D deterministic scheme)
+// Stage 2 as a plain column sum: out[i] = sum over the group rows of P[.][i],
+// i < count
 __global__ void __launch_bounds__(256)
-sum_partials_kernel(const float* __restrict__ P, int nblk, int stride, int count, int accumulate,
-                    float* __restrict__ out) {
-  __shared__ float part[4][64];
-  const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int i = blockIdx.x * 64 + j;
-  float s = 0.f;
-  if (i < count)
-    for (int k = g; k < nblk; k += 4) s += P[(int64_t)k * stride + i];
-  part[g][j] = s;
-  __syncthreads();
-  if (g == 0 && i < count) {
-    const float t = ((part[0][j] + part[1][j]) + part[2][j]) + part[3][j];
-    out[i] = accumulate ? out[i] + t : t;
-  }
+sum_partials_kernel(const float* __restrict__ P, int ngroups, int R, int stride, int count,
+                    int accumulate, float* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= count) return;
+  float t = 0.f;
+  for (int g = 0; g < ngroups; ++g) t += P[(int64_t)g * R * stride + i];
+  out[i] = accumulate ? out[i] + t : t;
 }
 
 // ------------------------------------------------------------ optimizer --
@@ -888,8 +945,13 @@ int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int
                        (const bf16*)dz, extents, B, H, W, workspace);
   IGM_CHECK_LAUNCH();
   const int entries = taps * kC * kC + kC;
-  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((entries + 63) / 64), dim3(256), 0, st, workspace,
-                     nblocks, taps, cout, accumulate, dW_hwio, db);
+  int R, G;
+  group_rows(nblocks, &R, &G);
+  hipLaunchKernelGGL(partials_group_sum_kernel, dim3((entries + 255) / 256, G), dim3(256), 0, st,
+                     workspace, nblocks, entries, entries, R);
+  IGM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((entries + 255) / 256), dim3(256), 0, st, workspace,
+                     G, R, taps, cout, accumulate, dW_hwio, db);
   IGM_CHECK_LAUNCH();
   return 0;
 }
@@ -907,11 +969,16 @@ int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const f
   IGM_CHECK_LAUNCH();
   const int n = 9 * kCin * kC + kC;
   // the layer-1 partial layout is [blk][1440 + 32] = [blk][n]: a plain column sum
-  hipLaunchKernelGGL(sum_partials_kernel, dim3((9 * kCin * kC + 63) / 64), dim3(256), 0, st,
-                     workspace, nblocks, n, 9 * kCin * kC, accumulate, dW_hwio);
+  int R, G;
+  group_rows(nblocks, &R, &G);
+  hipLaunchKernelGGL(partials_group_sum_kernel, dim3((n + 255) / 256, G), dim3(256), 0, st,
+                     workspace, nblocks, n, n, R);
+  IGM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(sum_partials_kernel, dim3((9 * kCin * kC + 255) / 256), dim3(256), 0, st,
+                     workspace, G, R, n, 9 * kCin * kC, accumulate, dW_hwio);
   IGM_CHECK_LAUNCH();
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, workspace + 9 * kCin * kC,
-                     nblocks, n, kC, accumulate, db);
+                     G, R, n, kC, accumulate, db);
   IGM_CHECK_LAUNCH();
   return 0;
 }

@@ -75,10 +75,13 @@ class Batcher:
         self.pool, self.batch, self.dev = pool, batch, dev
         self.buckets = max(1, min(int(buckets), batch))
         self.rng = np.random.default_rng(seed)
+        self.pixels = self.padded = 0  # valid and padded pixels served so far
 
     def _pack(self, parts):
         H = max(p.shape[0] for p in parts)
         W = max(p.shape[1] for p in parts)
+        self.pixels += sum(p.shape[0] * p.shape[1] for p in parts)
+        self.padded += len(parts) * H * W
         X = torch.zeros((len(parts), H, W, 5), dtype=torch.float32, device=self.dev)
         ext = torch.empty((len(parts), 2), dtype=torch.int32, pin_memory=True)
         for b, p in enumerate(parts):
@@ -109,9 +112,12 @@ class CropBatcher:
         for f in fields:
             assert f.shape[0] >= self.crop and f.shape[1] >= self.crop, "crop larger than a field"
         self.rng = np.random.default_rng(seed)
+        self.pixels = self.padded = 0
 
     def next(self):
         c = self.crop
+        self.pixels += self.batch * c * c
+        self.padded += self.batch * c * c
         X = torch.empty((self.batch, c, c, 5), dtype=torch.float32, device=self.dev)
         for b in range(self.batch):
             f = self.fields[int(self.rng.integers(len(self.fields)))]
@@ -216,6 +222,11 @@ def train(cnn, batcher, dzeta, phys, steps=None, seconds=None, lr=1e-4, lr_end=N
                     avg.swap()
             hist.append(tuple(rec))
             log(msg)
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    px, pad = getattr(batcher, "pixels", 0), getattr(batcher, "padded", 0)
+    log(f"trained {s} steps in {dt:.1f}s: {dt / max(1, s) * 1e3:.2f} ms/step, {px / dt:.4g} patch pixels/s "
+        f"({pad / dt:.4g} with the padding, {pad / max(1, px):.2f}x)")
     return hist
 
 
