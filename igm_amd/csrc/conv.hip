@@ -457,30 +457,34 @@ head_bwd_kernel(const float* __restrict__ dY, const bf16* __restrict__ a16,
 
 // ------------------------------------------------------------ wgrad MFMA --
 // dW[tap][co][ci] = sum_p A[p + delta_tap][ci] * dZ[p][co],  db[co] = sum_p dZ[p][co].
-// Persistent blocks over tiles of 1 row x 64 pixels; TAPS = 9 (3x3) or 1 (1x1).
-// Wave w owns the output blocks (co-block w&1, ci-block w>>1) of all taps.
-// LDS holds dZ and the activation rows transposed ([channel][pixel]), one
-// copy each, written as packed 2-pixel dwords.  The x-shift s of a tap is
-// made in registers from two aligned 16-byte reads: s=0 is the first read,
-// s=2 a dword select, s=1 a 16-bit funnel shift (v_alignbit).
-// partials: [gridDim.x][TAPS*1024 + 32] fp32.
+// Persistent blocks over tiles of kWgRows rows x 64 pixels; TAPS = 9 (3x3)
+// or 1 (1x1).  Wave w owns the output blocks (co-block w&1, ci-block w>>1)
+// of all taps.  LDS holds the tile's dZ rows and its activation rows (with a
+// one-row halo for 3x3) transposed ([channel][pixel]), written as packed
+// 2-pixel dwords; the halo rows are loaded once for the kWgRows rows.  The
+// x-shift s of a tap is made in registers from two aligned 16-byte reads:
+// s=0 is the first read, s=2 a dword select, s=1 a 16-bit funnel shift
+// (v_alignbit).  partials: [gridDim.x][TAPS*1024 + 32] fp32.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+constexpr int kWgRows = 4;  // dZ rows per tile (8 measured slower on c4 batches)
 
 template <int TAPS>
 __global__ void __launch_bounds__(256)
 wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
                   const int32_t* __restrict__ ext, int B, int H, int W, float* __restrict__ partials) {
-  constexpr int ROWS = TAPS == 9 ? 3 : 1;
+  constexpr int AROWS = TAPS == 9 ? kWgRows + 2 : kWgRows;  // activation rows in LDS
   constexpr int NT = TAPS;  // accumulators per wave
-  __shared__ __attribute__((aligned(16))) uint32_t lds_dz[kC][kChStride / 2];
-  __shared__ __attribute__((aligned(16))) uint32_t lds_act[ROWS][kC][kChStride / 2];
+  __shared__ __attribute__((aligned(16))) uint32_t lds_dz[kWgRows][kC][kChStride / 2];
+  __shared__ __attribute__((aligned(16))) uint32_t lds_act[AROWS][kC][kChStride / 2];
   __shared__ float dbred[256][8];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cb = wave & 1, cib = wave >> 1;
   const int ntx = (W + kTileW - 1) / kTileW;
-  const int64_t ntiles = (int64_t)B * H * ntx;
+  const int nty = (H + kWgRows - 1) / kWgRows;
+  const int64_t ntiles = (int64_t)B * nty * ntx;
 
   f32x4 acc[NT];
 #pragma unroll
@@ -490,37 +494,41 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
   for (int e = 0; e < 8; ++e) dba[e] = 0.f;
 
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int b = (int)(tile / ((int64_t)H * ntx));
-    const int y = (int)((tile / ntx) % H);
+    const int b = (int)(tile / ((int64_t)nty * ntx));
+    const int y0 = (int)((tile / ntx) % nty) * kWgRows;
     const int x0 = (int)(tile % ntx) * kTileW;
     int ny, nx;
     extent_of(ext, b, H, W, ny, nx);
-    if (y >= ny || x0 >= nx) continue;  // block-uniform
+    if (y0 >= ny || x0 >= nx) continue;  // block-uniform
     const int64_t img = (int64_t)b * H * W;
     __syncthreads();
-    if (tid < 128) {  // dZ row y, pixel pairs (x0 + 2q, x0 + 2q + 1) -> lds_dz[ch][q]
-      const int q = tid >> 2, chunk = tid & 3;
+    // dZ rows y0 .. y0 + kWgRows - 1, pixel pairs (x0 + 2q, x0 + 2q + 1) ->
+    // lds_dz[row][ch][q]; thread tid always takes channel chunk tid & 3
+    for (int c = tid; c < kWgRows * 32 * 4; c += 256) {
+      const int chunk = c & 3, q = (c >> 2) & 31, row = c >> 7;
+      const int yd = y0 + row, xa = x0 + 2 * q;
       uint4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
-      const int xa = x0 + 2 * q;
-      if (xa < nx) v0 = *reinterpret_cast<const uint4*>(dz + (img + (int64_t)y * W + xa) * kC + chunk * 8);
-      if (xa + 1 < nx) v1 = *reinterpret_cast<const uint4*>(dz + (img + (int64_t)y * W + xa + 1) * kC + chunk * 8);
-      const uint32_t a[4] = {v0.x, v0.y, v0.z, v0.w}, c[4] = {v1.x, v1.y, v1.z, v1.w};
+      if (yd < ny) {
+        if (xa < nx) v0 = *reinterpret_cast<const uint4*>(dz + (img + (int64_t)yd * W + xa) * kC + chunk * 8);
+        if (xa + 1 < nx) v1 = *reinterpret_cast<const uint4*>(dz + (img + (int64_t)yd * W + xa + 1) * kC + chunk * 8);
+      }
+      const uint32_t a[4] = {v0.x, v0.y, v0.z, v0.w}, c2[4] = {v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         // channel 2i: low halves; channel 2i+1: high halves (bf16 bit patterns)
-        lds_dz[chunk * 8 + 2 * i][q] = (a[i] & 0xFFFFu) | (c[i] << 16);
-        lds_dz[chunk * 8 + 2 * i + 1][q] = (a[i] >> 16) | (c[i] & 0xFFFF0000u);
-        dba[2 * i] += __uint_as_float(a[i] << 16) + __uint_as_float(c[i] << 16);
-        dba[2 * i + 1] += __uint_as_float(a[i] & 0xFFFF0000u) + __uint_as_float(c[i] & 0xFFFF0000u);
+        lds_dz[row][chunk * 8 + 2 * i][q] = (a[i] & 0xFFFFu) | (c2[i] << 16);
+        lds_dz[row][chunk * 8 + 2 * i + 1][q] = (a[i] >> 16) | (c2[i] & 0xFFFF0000u);
+        dba[2 * i] += __uint_as_float(a[i] << 16) + __uint_as_float(c2[i] << 16);
+        dba[2 * i + 1] += __uint_as_float(a[i] & 0xFFFF0000u) + __uint_as_float(c2[i] & 0xFFFF0000u);
       }
     }
-    // activation rows (y-1 .. y+1 for 3x3), pixels x0-1+t, t = 0..65, as pairs
+    // activation rows (y0 - 1 .. y0 + kWgRows for 3x3), pixels x0-1+t, t = 0..65, as pairs
     constexpr int kPairs = (kTileW + 2) / 2;
-    for (int c = tid; c < ROWS * kPairs * 4; c += 256) {
+    for (int c = tid; c < AROWS * kPairs * 4; c += 256) {
       const int chunk = c & 3;
       const int q = (c >> 2) % kPairs;
       const int row = (c >> 2) / kPairs;
-      const int yy = (ROWS == 3) ? y - 1 + row : y;
+      const int yy = (TAPS == 9) ? y0 - 1 + row : y0 + row;
       const int xa = x0 - 1 + 2 * q;
       uint4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
       if (yy >= 0 && yy < ny) {
@@ -536,38 +544,37 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
     }
     __syncthreads();
 #pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-      const int base = (ks * 32 + 8 * (lane >> 4)) / 2;  // dword index of the 8-pixel group
-      const u32x4 a4 = *reinterpret_cast<const u32x4*>(&lds_dz[cb * 16 + (lane & 15)][base]);
-      const bf16x8 afr = as_bf16x8(a4);
+    for (int rr = 0; rr < kWgRows; ++rr) {
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) {
-        // activation pixels x0 - 1 + t with t = 2*base .. 2*base + 15
-        const uint32_t* src = &lds_act[r][cib * 16 + (lane & 15)][base];
-        const u32x4 c0 = *reinterpret_cast<const u32x4*>(src);
-        if constexpr (TAPS == 9) {
-          const u32x4 c1 = *reinterpret_cast<const u32x4*>(src + 4);
-          // s = 0: pixels x0 - 1 + p  -> t = p          (c0)
-          // s = 1: pixels x0 + p      -> t = p + 1      (16-bit funnel shift)
-          // s = 2: pixels x0 + 1 + p  -> t = p + 2      (dwords 1..4)
-          u32x4 s1, s2;
-          s1[0] = __builtin_amdgcn_alignbit(c0[1], c0[0], 16);
-          s1[1] = __builtin_amdgcn_alignbit(c0[2], c0[1], 16);
-          s1[2] = __builtin_amdgcn_alignbit(c0[3], c0[2], 16);
-          s1[3] = __builtin_amdgcn_alignbit(c1[0], c0[3], 16);
-          s2[0] = c0[1]; s2[1] = c0[2]; s2[2] = c0[3]; s2[3] = c1[0];
-          acc[r * 3 + 0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(c0), acc[r * 3 + 0], 0, 0, 0);
-          acc[r * 3 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(s1), acc[r * 3 + 1], 0, 0, 0);
-          acc[r * 3 + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(s2), acc[r * 3 + 2], 0, 0, 0);
-        } else {
-          // 1x1: pixel x0 + p -> t = p + 1
+      for (int ks = 0; ks < 2; ++ks) {
+        const int base = (ks * 32 + 8 * (lane >> 4)) / 2;  // dword index of the 8-pixel group
+        const u32x4 a4 = *reinterpret_cast<const u32x4*>(&lds_dz[rr][cb * 16 + (lane & 15)][base]);
+        const bf16x8 afr = as_bf16x8(a4);
+#pragma unroll
+        for (int r = 0; r < (TAPS == 9 ? 3 : 1); ++r) {
+          // activation pixels x0 - 1 + t with t = 2*base .. 2*base + 15, of
+          // activation row rr + r (dZ row rr, tap row r)
+          const uint32_t* src = &lds_act[rr + r][cib * 16 + (lane & 15)][base];
+          const u32x4 c0 = *reinterpret_cast<const u32x4*>(src);
           const u32x4 c1 = *reinterpret_cast<const u32x4*>(src + 4);
           u32x4 s1;
           s1[0] = __builtin_amdgcn_alignbit(c0[1], c0[0], 16);
           s1[1] = __builtin_amdgcn_alignbit(c0[2], c0[1], 16);
           s1[2] = __builtin_amdgcn_alignbit(c0[3], c0[2], 16);
           s1[3] = __builtin_amdgcn_alignbit(c1[0], c0[3], 16);
-          acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(s1), acc[0], 0, 0, 0);
+          if constexpr (TAPS == 9) {
+            // s = 0: pixels x0 - 1 + p  -> t = p          (c0)
+            // s = 1: pixels x0 + p      -> t = p + 1      (16-bit funnel shift)
+            // s = 2: pixels x0 + 1 + p  -> t = p + 2      (dwords 1..4)
+            u32x4 s2;
+            s2[0] = c0[1]; s2[1] = c0[2]; s2[2] = c0[3]; s2[3] = c1[0];
+            acc[r * 3 + 0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(c0), acc[r * 3 + 0], 0, 0, 0);
+            acc[r * 3 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(s1), acc[r * 3 + 1], 0, 0, 0);
+            acc[r * 3 + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(s2), acc[r * 3 + 2], 0, 0, 0);
+          } else {
+            // 1x1: pixel x0 + p -> t = p + 1
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, as_bf16x8(s1), acc[0], 0, 0, 0);
+          }
         }
       }
     }
@@ -582,14 +589,14 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
       const int co = cb * 16 + 4 * (lane >> 4) + e;
       P[(tap * kC + co) * kC + ci] = acc[tap][e];
     }
-  // partial db: threads tid < 128 hold channels (tid&3)*8 .. +8
+  // partial db: thread t holds channels (t&3)*8 .. +8
 #pragma unroll
   for (int e = 0; e < 8; ++e) dbred[tid][e] = dba[e];
   __syncthreads();
   if (tid < kC) {
     const int chunk = tid >> 3, e = tid & 7;
     float s = 0.f;
-    for (int t = chunk; t < 128; t += 4) s += dbred[t][e];
+    for (int t = chunk; t < 256; t += 4) s += dbred[t][e];
     P[TAPS * kC * kC + tid] = s;
   }
 }
