@@ -268,6 +268,26 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
     }
     // (the compute below is reached only through this path, after the wait
     // for the staged loads, so the compiler places no memory waits in it)
+    // MODE 1: the epilogue's activation signs are loaded here, before the
+    // barrier and so before the next tile's staging loads.  Loads complete
+    // in issue order, so the epilogue's wait for them leaves the staging
+    // loads in flight (issued after them, it would wait for those too).
+    uint2 auxv[4][2];
+    if constexpr (MODE == 1) {
+      const rsrc_t raux = make_rsrc(aux, (uint32_t)((int64_t)B * H * W * kC * 2));
+#pragma unroll
+      for (int pb = 0; pb < 4; ++pb) {
+        const int x = x0 + pb * 16 + (lane & 15);
+        const bool ok = y < H && x < W;
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+          const int co = cb * 16 + 4 * (lane >> 4);
+          const uint32_t off = ok ? (uint32_t)(((img + (int64_t)y * W + x) * kC + co) * 2) : kOOB;
+          const auto v = __builtin_amdgcn_raw_buffer_load_b64(raux, off, 0, 0);
+          auxv[pb][cb] = uint2{v[0], v[1]};
+        }
+      }
+    }
     __syncthreads();  // the previous tile's fragment reads are done
 #pragma unroll
     for (int k = 0; k < kPerThread; ++k) {
@@ -318,8 +338,7 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
 #pragma unroll
             for (int e = 0; e < 4; ++e) v[e] = leaky(acc[pb][cb][e] + bias_r[cb][e]);
           } else {
-            const uint2 g2 = *reinterpret_cast<const uint2*>(aux + po + co);
-            const bf16* g = reinterpret_cast<const bf16*>(&g2);
+            const bf16* g = reinterpret_cast<const bf16*>(&auxv[pb][cb]);
 #pragma unroll
             for (int e = 0; e < 4; ++e) v[e] = acc[pb][cb][e] * (bf2f(g[e]) > 0.f ? 1.f : 0.01f);
           }
