@@ -97,8 +97,10 @@ ICEFLOW = {
         },
         # build-specific polish of every emulated field: `its` plain-solver
         # iterations warm-started from it, step lr for flat_frac of them,
-        # then a geometric decay to lr_end (0 its: off)
-        "refine": {"its": 0, "lr": 3.0, "flat_frac": 0.3, "lr_end": 1.0e-3},
+        # then a geometric decay to lr_end (0 its: off); beta1, beta2 of its
+        # Adam (the Keras defaults unless set)
+        "refine": {"its": 0, "lr": 3.0, "flat_frac": 0.3, "lr_end": 1.0e-3, "beta1": 0.9,
+                   "beta2": 0.999},
     },
 }
 

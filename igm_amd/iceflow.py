@@ -189,18 +189,22 @@ def refine_emulated(cfg, state):
     """Polish the emulated field: ``refine.its`` plain-solver iterations
     warm-started from state.U, state.V with a fresh legacy-Adam state, step
     ``refine.lr`` for the first ``flat_frac`` of them, then a geometric decay
-    to ``lr_end``; the outputs are rebuilt from the polished field.  A
-    network whose error is a few percent reaches the solver's accuracy in a
-    small fraction of the iterations a solve from zero needs."""
+    to ``lr_end``, with Adam's ``beta1``, ``beta2``; the outputs are
+    rebuilt from the polished field.  A network whose error is a few percent
+    reaches the solver's accuracy in a small fraction of the iterations a
+    solve from zero needs."""
     ice = cfg.processes.iceflow
-    rf = ice.emulator.refine
-    n = int(rf.its)
+    d = cfgmod.ICEFLOW["emulator"]["refine"]
+    rf = {k: cfgmod.get(ice, "emulator.refine." + k, v) for k, v in d.items()}
+    n = int(rf["its"])
     if n <= 0:
         return
     op = energy_operator(cfg, state)
-    n_flat = int(float(rf.flat_frac) * n)
-    res = slv.solve(op, state.U, state.V, n, slv.LegacyAdamState(lr=float(rf.lr)), early_stopping=False,
-                    lr_schedule=slv.adam_schedule(float(rf.lr), n_flat, n - n_flat, float(rf.lr_end)))
+    lr = float(rf["lr"])
+    n_flat = int(float(rf["flat_frac"]) * n)
+    opt = slv.LegacyAdamState(lr=lr, beta1=float(rf["beta1"]), beta2=float(rf["beta2"]))
+    res = slv.solve(op, state.U, state.V, n, opt, early_stopping=False,
+                    lr_schedule=slv.adam_schedule(lr, n_flat, n - n_flat, float(rf["lr_end"])))
     out = velocity_outputs(res.U, res.V, state.thk, state.vert_weight, int(ice.numerics.Nz))
     _set_outputs(cfg, state, out)
     state.COST_REFINED = float(res.costs[-1])
