@@ -47,6 +47,19 @@ __device__ int solve_prologue(const PassArgs<float>& a, bool block0, int* it0) {
   IgmEarlyStop st = *a.es_in;
   const int f_prev = st.iters_done;
   const int c_prev = a.prev_partials != nullptr ? pass_iterations(st, f_prev, a.nbitmax) : 0;
+  // The previous pass's COSTs can raise `wait` by at most one each (the
+  // first only sets the best when there is none).  If even then wait + 1
+  // stays below patience, the rule stops nowhere in that pass and cannot stop
+  // at this pass's first iteration: the count below is pass_iterations' for
+  // any COSTs.  Only block 0, which writes the state and the COST history,
+  // then needs the sums; every other block starts its work at once.
+  const int max_wait = st.wait + (st.has_best ? c_prev : (c_prev > 0 ? c_prev - 1 : 0));
+  if (!block0 && !st.stopped && max_wait + 1 < st.patience) {
+    const int f = f_prev + c_prev;
+    *it0 = f;
+    if (a.finalize_only || f >= a.nbitmax) return 0;
+    return f + 1 < a.nbitmax ? 2 : 1;
+  }
   for (int t = 0; t < c_prev; ++t) {
     double r[3];
     block_sum_partials(a.prev_partials + 3 * t, a.prev_nblk, 6, r);
