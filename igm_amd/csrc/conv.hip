@@ -184,12 +184,14 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nty = (H + kTileH - 1) / kTileH;
   const int ntx = (W + kTileW - 1) / kTileW;
-  const int64_t ntiles = (int64_t)B * ntx * nty;
+  // (32-bit tile indices: the host keeps the activations below 2 GiB, so
+  // there are fewer than 2^25 tiles)
+  const int ntiles = B * ntx * nty;
   // XCD-contiguous runs of tiles: vertically adjacent tiles (ntx apart in
   // the x-fastest order) then share one L2 for their halo rows
-  const int64_t t0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * per;
+  const int t0 = xcd_remap(blockIdx.x, gridDim.x) * per;
   if (t0 >= ntiles) return;
-  const int64_t t1 = t0 + per < ntiles ? t0 + per : ntiles;
+  const int t1 = t0 + per < ntiles ? t0 + per : ntiles;
 
   // weight fragments: A operand (16 out-channels x 32 in-channels per tap)
   bf16x8 wf[9][2];
@@ -217,11 +219,11 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
     int b, y0, x0, ny, nx;
     bool valid;
   };
-  auto tile_of = [&](int64_t ti) {
+  auto tile_of = [&](int ti) {
     Tile t;
-    t.x0 = (int)(ti % ntx) * kTileW;
-    t.y0 = (int)((ti / ntx) % nty) * kTileH;
-    t.b = (int)(ti / ((int64_t)nty * ntx));
+    t.x0 = (ti % ntx) * kTileW;
+    t.y0 = ((ti / ntx) % nty) * kTileH;
+    t.b = ti / (nty * ntx);
     extent_of(ext, t.b, H, W, t.ny, t.nx);
     t.valid = t.y0 < t.ny && t.x0 < t.nx;
     return t;
@@ -247,7 +249,7 @@ conv3x3_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
 
   Tile cur = tile_of(t0);
   issue_loads(cur, true);
-  for (int64_t ti = t0; ti < t1; ++ti) {
+  for (int ti = t0; ti < t1; ++ti) {
     const int64_t img = (int64_t)cur.b * H * W;
     const int y = cur.y0 + wave;
     const int x0 = cur.x0;
@@ -483,7 +485,8 @@ head_bwd_kernel(const float* __restrict__ dY, const bf16* __restrict__ a16,
 // 2-pixel dwords; the halo rows are loaded once for the kWgRows rows.  The
 // x-shift s of a tap is made in registers from two aligned 16-byte reads:
 // s=0 is the first read, s=2 a dword select, s=1 a 16-bit funnel shift
-// (v_alignbit).  partials: [gridDim.x][TAPS*1024 + 32] fp32.
+// (v_alignbit).  A block loads its next tile into registers before the
+// current tile's MFMAs.  partials: [gridDim.x][TAPS*1024 + 32] fp32.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
@@ -503,7 +506,7 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
   const int cb = wave & 1, cib = wave >> 1;
   const int ntx = (W + kTileW - 1) / kTileW;
   const int nty = (H + kWgRows - 1) / kWgRows;
-  const int64_t ntiles = (int64_t)B * nty * ntx;
+  const int ntiles = B * nty * ntx;  // < 2^25: activations below 2 GiB
 
   f32x4 acc[NT];
 #pragma unroll
@@ -512,56 +515,114 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
 #pragma unroll
   for (int e = 0; e < 8; ++e) dba[e] = 0.f;
 
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int b = (int)(tile / ((int64_t)nty * ntx));
-    const int y0 = (int)((tile / ntx) % nty) * kWgRows;
-    const int x0 = (int)(tile % ntx) * kTileW;
-    int ny, nx;
-    extent_of(ext, b, H, W, ny, nx);
-    if (y0 >= ny || x0 >= nx) continue;  // block-uniform
-    const int64_t img = (int64_t)b * H * W;
-    __syncthreads();
-    // dZ rows y0 .. y0 + kWgRows - 1, pixel pairs (x0 + 2q, x0 + 2q + 1) ->
-    // lds_dz[row][ch][q]; thread tid always takes channel chunk tid & 3
-    for (int c = tid; c < kWgRows * 32 * 4; c += 256) {
+  // Staging of a tile: dZ rows y0 .. y0 + kWgRows - 1 and activation rows
+  // (y0 - 1 .. y0 + kWgRows for 3x3), each as 16-B channel chunks of pixel
+  // pairs, in registers.  The next tile's loads are issued before the
+  // current tile's MFMAs, so they are in flight during them; masked lanes
+  // (outside the entry's extent, or no next tile) read zeros.
+  constexpr int kPairs = (kTileW + 2) / 2;
+  constexpr int kDzItems = kWgRows * 32 * 4;
+  constexpr int kDzPer = (kDzItems + 255) / 256;
+  constexpr int kActItems = AROWS * kPairs * 4;
+  constexpr int kActPer = (kActItems + 255) / 256;
+  const uint32_t vbytes = (uint32_t)((int64_t)B * H * W * kC * 2);
+  const rsrc_t rdz = make_rsrc(dz, vbytes), ract = make_rsrc(act, vbytes);
+  uint4 sdz[kDzPer][2], sact[kActPer][2];
+  struct Tile {
+    int b, y0, x0, ny, nx;
+    bool valid;
+  };
+  auto tile_of = [&](int t) {
+    Tile r;
+    r.b = t / (nty * ntx);
+    r.y0 = ((t / ntx) % nty) * kWgRows;
+    r.x0 = (t % ntx) * kTileW;
+    extent_of(ext, r.b, H, W, r.ny, r.nx);
+    r.valid = r.y0 < r.ny && r.x0 < r.nx;
+    return r;
+  };
+  auto ld16 = [&](rsrc_t r, bool ok, int64_t pix, int chunk) {
+    const uint32_t off = ok ? (uint32_t)((pix * kC + chunk * 8) * 2) : kOOB;
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return uint4{v[0], v[1], v[2], v[3]};
+  };
+  auto issue_loads = [&](const Tile& t, bool live) {
+    const int64_t img = (int64_t)t.b * H * W;
+    const bool on = live && t.valid;
+#pragma unroll
+    for (int k = 0; k < kDzPer; ++k) {
+      const int c = tid + 256 * k;
       const int chunk = c & 3, q = (c >> 2) & 31, row = c >> 7;
-      const int yd = y0 + row, xa = x0 + 2 * q;
-      uint4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
-      if (yd < ny) {
-        if (xa < nx) v0 = *reinterpret_cast<const uint4*>(dz + (img + (int64_t)yd * W + xa) * kC + chunk * 8);
-        if (xa + 1 < nx) v1 = *reinterpret_cast<const uint4*>(dz + (img + (int64_t)yd * W + xa + 1) * kC + chunk * 8);
-      }
-      const uint32_t a[4] = {v0.x, v0.y, v0.z, v0.w}, c2[4] = {v1.x, v1.y, v1.z, v1.w};
+      const int yd = t.y0 + row, xa = t.x0 + 2 * q;
+      const bool rok = on && c < kDzItems && yd < t.ny;
+      const int64_t pix = img + (int64_t)yd * W + xa;
+      sdz[k][0] = ld16(rdz, rok && xa < t.nx, pix, chunk);
+      sdz[k][1] = ld16(rdz, rok && xa + 1 < t.nx, pix + 1, chunk);
+    }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        // channel 2i: low halves; channel 2i+1: high halves (bf16 bit patterns)
-        lds_dz[row][chunk * 8 + 2 * i][q] = (a[i] & 0xFFFFu) | (c2[i] << 16);
-        lds_dz[row][chunk * 8 + 2 * i + 1][q] = (a[i] >> 16) | (c2[i] & 0xFFFF0000u);
-        dba[2 * i] += __uint_as_float(a[i] << 16) + __uint_as_float(c2[i] << 16);
-        dba[2 * i + 1] += __uint_as_float(a[i] & 0xFFFF0000u) + __uint_as_float(c2[i] & 0xFFFF0000u);
+    for (int k = 0; k < kActPer; ++k) {
+      const int c = tid + 256 * k;
+      const int chunk = c & 3, q = (c >> 2) % kPairs, row = (c >> 2) / kPairs;
+      const int yy = (TAPS == 9) ? t.y0 - 1 + row : t.y0 + row;
+      const int xa = t.x0 - 1 + 2 * q;
+      const bool rok = on && c < kActItems && yy >= 0 && yy < t.ny;
+      const int64_t pix = img + (int64_t)yy * W + xa;
+      sact[k][0] = ld16(ract, rok && xa >= 0 && xa < t.nx, pix, chunk);
+      sact[k][1] = ld16(ract, rok && xa + 1 >= 0 && xa + 1 < t.nx, pix + 1, chunk);
+    }
+  };
+  // staged registers -> LDS, channel-major with pixel pairs per dword:
+  // lds_dz[row][ch][q] holds pixels (x0 + 2q, x0 + 2q + 1), lds_act[row][ch][q]
+  // pixels (x0 - 1 + 2q, x0 + 2q); thread tid always takes channel chunk
+  // tid & 3, and sums its dZ channels into the bias gradient
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int k = 0; k < kDzPer; ++k) {
+      const int c = tid + 256 * k;
+      if (c < kDzItems) {
+        const int chunk = c & 3, q = (c >> 2) & 31, row = c >> 7;
+        const uint32_t a[4] = {sdz[k][0].x, sdz[k][0].y, sdz[k][0].z, sdz[k][0].w};
+        const uint32_t c2[4] = {sdz[k][1].x, sdz[k][1].y, sdz[k][1].z, sdz[k][1].w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          // channel 2i: low halves; channel 2i+1: high halves (bf16 bit patterns)
+          lds_dz[row][chunk * 8 + 2 * i][q] = (a[i] & 0xFFFFu) | (c2[i] << 16);
+          lds_dz[row][chunk * 8 + 2 * i + 1][q] = (a[i] >> 16) | (c2[i] & 0xFFFF0000u);
+          dba[2 * i] += __uint_as_float(a[i] << 16) + __uint_as_float(c2[i] << 16);
+          dba[2 * i + 1] += __uint_as_float(a[i] & 0xFFFF0000u) + __uint_as_float(c2[i] & 0xFFFF0000u);
+        }
       }
     }
-    // activation rows (y0 - 1 .. y0 + kWgRows for 3x3), pixels x0-1+t, t = 0..65, as pairs
-    constexpr int kPairs = (kTileW + 2) / 2;
-    for (int c = tid; c < AROWS * kPairs * 4; c += 256) {
-      const int chunk = c & 3;
-      const int q = (c >> 2) % kPairs;
-      const int row = (c >> 2) / kPairs;
-      const int yy = (TAPS == 9) ? y0 - 1 + row : y0 + row;
-      const int xa = x0 - 1 + 2 * q;
-      uint4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
-      if (yy >= 0 && yy < ny) {
-        if (xa >= 0 && xa < nx) v0 = *reinterpret_cast<const uint4*>(act + (img + (int64_t)yy * W + xa) * kC + chunk * 8);
-        if (xa + 1 >= 0 && xa + 1 < nx) v1 = *reinterpret_cast<const uint4*>(act + (img + (int64_t)yy * W + xa + 1) * kC + chunk * 8);
-      }
-      const uint32_t a[4] = {v0.x, v0.y, v0.z, v0.w}, c2[4] = {v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        lds_act[row][chunk * 8 + 2 * i][q] = (a[i] & 0xFFFFu) | (c2[i] << 16);
-        lds_act[row][chunk * 8 + 2 * i + 1][q] = (a[i] >> 16) | (c2[i] & 0xFFFF0000u);
+    for (int k = 0; k < kActPer; ++k) {
+      const int c = tid + 256 * k;
+      if (c < kActItems) {
+        const int chunk = c & 3, q = (c >> 2) % kPairs, row = (c >> 2) / kPairs;
+        const uint32_t a[4] = {sact[k][0].x, sact[k][0].y, sact[k][0].z, sact[k][0].w};
+        const uint32_t c2[4] = {sact[k][1].x, sact[k][1].y, sact[k][1].z, sact[k][1].w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          lds_act[row][chunk * 8 + 2 * i][q] = (a[i] & 0xFFFFu) | (c2[i] << 16);
+          lds_act[row][chunk * 8 + 2 * i + 1][q] = (a[i] >> 16) | (c2[i] & 0xFFFF0000u);
+        }
       }
     }
-    __syncthreads();
+  };
+
+  int tile = blockIdx.x;
+  Tile cur = tile_of(tile < ntiles ? tile : 0);
+  issue_loads(cur, tile < ntiles);
+  for (; tile < ntiles; tile += gridDim.x) {
+    const bool valid = cur.valid;  // block-uniform
+    if (valid) {
+      __syncthreads();  // the previous tile's fragment reads are done
+      store_tile();
+      __syncthreads();
+    }
+    const int nt = tile + gridDim.x;
+    cur = tile_of(nt < ntiles ? nt : tile);
+    issue_loads(cur, nt < ntiles);
+    if (!valid) continue;
 #pragma unroll
     for (int rr = 0; rr < kWgRows; ++rr) {
 #pragma unroll
@@ -715,11 +776,11 @@ conv_in_wgrad_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
     for (int q = 0; q < 4; ++q) acc[j][q] = 0.f;
   }
   const int ntx = (W + kWgTileW - 1) / kWgTileW, nty = (H + kWgTileH - 1) / kWgTileH;
-  const int64_t ntiles = (int64_t)B * nty * ntx;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int b = (int)(tile / ((int64_t)nty * ntx));
-    const int y0 = (int)((tile / ntx) % nty) * kWgTileH;
-    const int x0 = (int)(tile % ntx) * kWgTileW;
+  const int ntiles = B * nty * ntx;  // < 2^25: layer-1 gradient below 2 GiB
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int b = tile / (nty * ntx);
+    const int y0 = ((tile / ntx) % nty) * kWgTileH;
+    const int x0 = (tile % ntx) * kWgTileW;
     int ny, nx;
     extent_of(ext, b, H, W, ny, nx);
     if (y0 >= ny || x0 >= nx) continue;
@@ -962,6 +1023,7 @@ int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int
   IGM_CHECK_ARG(taps == 9 || taps == 1, "taps must be 9 or 1");
   IGM_CHECK_ARG(cout >= 1 && cout <= kC, "cout must be in [1, 32]");
   IGM_CHECK_ARG(nblocks >= 1 && B >= 1 && H >= 1 && W >= 1, "bad sizes");
+  IGM_CHECK_ARG((int64_t)B * H * W * kC * 2 < (int64_t)kOOB, "activations above 2 GiB (32-bit buffer offsets)");
   hipStream_t st = (hipStream_t)stream;
   if (taps == 9)
     hipLaunchKernelGGL(wgrad_mfma_kernel<9>, dim3(nblocks), dim3(256), 0, st, (const bf16*)act,
@@ -989,6 +1051,7 @@ int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const f
                       void* stream) {
   IGM_CHECK_ARG(f0 && f1 && f2 && f3 && f4 && dz1 && workspace && dW_hwio && db, "null pointer");
   IGM_CHECK_ARG(nblocks >= 1 && B >= 1 && H >= 1 && W >= 1, "bad sizes");
+  IGM_CHECK_ARG((int64_t)B * H * W * kC * 2 < (int64_t)kOOB, "activations above 2 GiB (32-bit buffer offsets)");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(conv_in_wgrad_kernel, dim3(nblocks), dim3(256), 0, st, f0, f1, f2, f3, f4, fb,
                      fr, fc, extents, B, H, W, (const bf16*)dz1, workspace);
