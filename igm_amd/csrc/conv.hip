@@ -57,18 +57,16 @@ __device__ __forceinline__ void extent_of(const int32_t* ext, int b, int H, int 
 }
 
 // ------------------------------------------------------------ layer 1 --
-// One thread per output pixel; weights HWIO [3][3][5][32] fp32 in LDS.
+// One thread per output pixel.  The 45 inputs of its 3x3 window are loaded
+// up front; the weights (HWIO [3][3][5][32] fp32) are wave-uniform, so they
+// come through scalar loads (read-only, no LDS) as the FMA operand.  Each
+// output channel accumulates bias, then taps 0..8, then fields 0..4.
 __global__ void __launch_bounds__(256)
 conv_in_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
                const float* __restrict__ f2, const float* __restrict__ f3,
                const float* __restrict__ f4, int64_t fb, int64_t fr, int64_t fc,
                const int32_t* __restrict__ ext, int H, int W, const float* __restrict__ w,
                const float* __restrict__ bias, bf16* __restrict__ out) {
-  __shared__ float ws[9 * kCin * kC];
-  __shared__ float bs[kC];
-  for (int i = threadIdx.x; i < 9 * kCin * kC; i += blockDim.x) ws[i] = w[i];
-  if (threadIdx.x < kC) bs[threadIdx.x] = bias[threadIdx.x];
-  __syncthreads();
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   const int b = blockIdx.z;
@@ -83,32 +81,32 @@ conv_in_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
     return;
   }
   const float* fs[kCin] = {f0, f1, f2, f3, f4};
+  float v[9][kCin];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int yy = y + t / 3 - 1, xx = x + t % 3 - 1;
+    const bool in = yy >= 0 && yy < ny && xx >= 0 && xx < nx;
+    const int64_t off = (int64_t)b * fb + (int64_t)yy * fr + (int64_t)xx * fc;
+#pragma unroll
+    for (int ci = 0; ci < kCin; ++ci) v[t][ci] = in ? fs[ci][off] : 0.f;
+  }
   float acc[kC];
 #pragma unroll
-  for (int c = 0; c < kC; ++c) acc[c] = bs[c];
-#pragma unroll 1
-  for (int r = 0; r < 3; ++r) {
-    const int yy = y + r - 1;
-#pragma unroll 1
-    for (int s = 0; s < 3; ++s) {
-      const int xx = x + s - 1;
-      const bool in = yy >= 0 && yy < ny && xx >= 0 && xx < nx;
-      const int64_t off = (int64_t)b * fb + (int64_t)yy * fr + (int64_t)xx * fc;
+  for (int c = 0; c < kC; ++c) acc[c] = bias[c];
 #pragma unroll
-      for (int ci = 0; ci < kCin; ++ci) {
-        const float v = in ? fs[ci][off] : 0.f;
-        const float* wr = ws + ((r * 3 + s) * kCin + ci) * kC;
+  for (int t = 0; t < 9; ++t)
 #pragma unroll
-        for (int c = 0; c < kC; ++c) acc[c] += v * wr[c];
-      }
+    for (int ci = 0; ci < kCin; ++ci) {
+      const float* wr = w + (t * kCin + ci) * kC;
+#pragma unroll
+      for (int c = 0; c < kC; ++c) acc[c] += v[t][ci] * wr[c];
     }
-  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    bf16x8 v;
+    bf16x8 h;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = f2bf(leaky(acc[q * 8 + e]));
-    reinterpret_cast<bf16x8*>(o)[q] = v;
+    for (int e = 0; e < 8; ++e) h[e] = f2bf(leaky(acc[q * 8 + e]));
+    reinterpret_cast<bf16x8*>(o)[q] = h;
   }
 }
 
@@ -431,8 +429,14 @@ __global__ void __launch_bounds__(256)
 head_bwd_kernel(const float* __restrict__ dY, const bf16* __restrict__ a16,
                 const float* __restrict__ w17, const int32_t* __restrict__ ext, int H, int W,
                 int cout, bf16* __restrict__ dz16, bf16* __restrict__ dYb) {
-  __shared__ float ws[kC * kC];
-  for (int i = threadIdx.x; i < kC * cout; i += blockDim.x) ws[i] = w17[i];
+  // ws[ci][co] = w17[ci][co], zero for co >= cout: the loops below run over
+  // whole groups of 4 output channels with static register indices (the
+  // padded terms add exact zeros)
+  __shared__ __attribute__((aligned(16))) float ws[kC][kC];
+  for (int i = threadIdx.x; i < kC * kC; i += blockDim.x) {
+    const int ci = i / kC, co = i % kC;
+    ws[ci][co] = co < cout ? w17[ci * cout + co] : 0.f;
+  }
   __syncthreads();
   const int64_t npx = (int64_t)H * W;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -454,7 +458,24 @@ head_bwd_kernel(const float* __restrict__ dY, const bf16* __restrict__ a16,
   }
   float d[kC];
 #pragma unroll
-  for (int c = 0; c < kC; ++c) d[c] = (c < cout && valid) ? dY[q * cout + c] : 0.f;
+  for (int c = 0; c < kC; ++c) d[c] = c < cout ? dY[q * cout + c] : 0.f;
+  // s[ci] = sum over co = 0 .. cout-1, in order, of d[co] * w[ci][co]
+  float s[kC];
+#pragma unroll
+  for (int ci = 0; ci < kC; ++ci) s[ci] = 0.f;
+#pragma unroll
+  for (int g = 0; g < kC / 4; ++g) {
+    if (4 * g < cout) {  // wave-uniform
+#pragma unroll
+      for (int ci = 0; ci < kC; ++ci) {
+        const float4 wv = *reinterpret_cast<const float4*>(&ws[ci][4 * g]);
+        s[ci] += d[4 * g] * wv.x;
+        s[ci] += d[4 * g + 1] * wv.y;
+        s[ci] += d[4 * g + 2] * wv.z;
+        s[ci] += d[4 * g + 3] * wv.w;
+      }
+    }
+  }
   bf16x8 outv[4], outy[4];
 #pragma unroll
   for (int c4 = 0; c4 < 4; ++c4) {
@@ -462,10 +483,7 @@ head_bwd_kernel(const float* __restrict__ dY, const bf16* __restrict__ a16,
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int ci = c4 * 8 + e;
-      float s = 0.f;
-      for (int co = 0; co < cout; ++co) s += d[co] * ws[ci * cout + co];
-      s *= (bf2f(av[e]) > 0.f ? 1.f : 0.01f);
-      outv[c4][e] = f2bf(valid ? s : 0.f);
+      outv[c4][e] = f2bf(s[ci] * (bf2f(av[e]) > 0.f ? 1.f : 0.01f));
       outy[c4][e] = f2bf(d[ci]);
     }
   }
