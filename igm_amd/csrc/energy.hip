@@ -28,6 +28,9 @@
 
 namespace igm {
 
+// four dwords, the operand of the 16-byte buffer loads and stores
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
+
 namespace {
 thread_local std::string g_last_error;
 }
@@ -141,7 +144,11 @@ __device__ void finalize_entry(const double* p, int nblk, int count, double* out
 }
 
 // INTER = 0: planar (Nz, Ny, Nx) with separate U, V buffers;
-// INTER = 1: node-interleaved Y (Ny, Nx, 2Nz), V = Y + Nz.
+// INTER = 1: node-interleaved Y (Ny, Nx, 2Nz), V = Y + Nz, one level per access;
+// INTER = 2: the same layout, float32, even Nz, 16-B aligned: a node's 2 Nz
+//   values (80 B at Nz 10) move as Nz/2 16-B accesses per lane, and mode 1
+//   stores the gradient of a node after its level sweep the same way (the
+//   one-level accesses of a 64-lane wave touch 64 cache lines 80 B apart).
 template <typename T, int NZ, int MODE, int INTER>
 __global__ void __launch_bounds__(kPassBlock)
 energy_pass_kernel(const PassArgs<T> a) {
@@ -157,6 +164,8 @@ energy_pass_kernel(const PassArgs<T> a) {
     stopped = stopped || st;
   }
   constexpr uint32_t ES = sizeof(T);
+  constexpr bool kInter = INTER != 0;
+  constexpr bool kVecNode = INTER == 2 && NZ % 2 == 0 && sizeof(T) == 4;
 
   if (!stopped && wid < a.nstrips * a.nrowblk) {
     const int strip = wid % a.nstrips;
@@ -171,7 +180,7 @@ energy_pass_kernel(const PassArgs<T> a) {
     const rsrc_t rC = make_rsrc(a.coef, a.coef_bytes);
     const uint32_t bbase = (uint32_t)b * a.bs_bytes;
     const uint32_t cbase = (uint32_t)b * (uint32_t)(kPlanes * a.plane) * ES;
-    const uint32_t kstep = INTER ? ES : (uint32_t)a.plane * ES;   // bytes between levels
+    const uint32_t kstep = kInter ? ES : (uint32_t)a.plane * ES;   // bytes between levels
     const T hh = a.hscale[b];
     const T hh2 = hh * hh;
     const T h05 = T(0.5) * hh2, h025 = T(0.25) * hh2;
@@ -179,17 +188,33 @@ energy_pass_kernel(const PassArgs<T> a) {
     // byte offset of node (row, ic), level 0 -- or kOOB outside the grid
     auto node_off = [&](int row, bool extra) -> uint32_t {
       const bool ok = extra && col_ok && row >= 0 && row < a.Ny;
-      const uint32_t e = (uint32_t)(row * a.Nx + ic) * (INTER ? 2u * NZ : 1u);
+      const uint32_t e = (uint32_t)(row * a.Nx + ic) * (kInter ? 2u * NZ : 1u);
       return ok ? bbase + e * ES : kOOB;
     };
     // node row `row` of U and V (rows this wave never uses -- past j1 --
     // read nothing: the last step's prefetch would be wasted bytes)
     auto load_row = [&](T (&u)[NZ], T (&v)[NZ], int row) {
       const uint32_t o = node_off(row, row <= j1);
+      if constexpr (kVecNode) {
+        // U levels then V levels of the node: V = U + Nz, all through rU
+        float f[2 * NZ];
 #pragma unroll
-      for (int k = 0; k < NZ; ++k) {
-        u[k] = bload(rU, o, k * kstep, T());
-        v[k] = bload(rV, o, k * kstep, T());
+        for (int i = 0; i < NZ / 2; ++i) {
+          const auto x = __builtin_amdgcn_raw_buffer_load_b128(rU, o, 16 * i, 0);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) f[4 * i + e] = __uint_as_float(x[e]);
+        }
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          u[k] = f[k];
+          v[k] = f[NZ + k];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          u[k] = bload(rU, o, k * kstep, T());
+          v[k] = bload(rV, o, k * kstep, T());
+        }
       }
     };
 
@@ -252,7 +277,10 @@ energy_pass_kernel(const PassArgs<T> a) {
       };
       // output of node (jc, ic), level k (masked stores)
       auto emit = [&](int k, T g_u, T g_v) {
-        if constexpr (MODE == 1) {
+        if constexpr (MODE == 1 && kVecNode) {
+          gu_[k] = g_u;  // stored as whole nodes after the sweep
+          gv_[k] = g_v;
+        } else if constexpr (MODE == 1) {
           bstore(g_u, make_rsrc(a.gU, a.vel_bytes), on, k * kstep);
           bstore(g_v, make_rsrc(a.gV, a.vel_bytes), on, k * kstep);
         } else if constexpr (kDeferAdam) {
@@ -265,6 +293,22 @@ energy_pass_kernel(const PassArgs<T> a) {
         }
       };
       const CellEnergy<T> e = level_sweep<T, NZ>(a, hh2, h05, h025, up, vp, uc, vc, cc, cu, cv, emit);
+      if constexpr (MODE == 1 && kVecNode) {
+        // the node's gradient: gU levels then gV levels (gV = gU + Nz)
+        const rsrc_t rG = make_rsrc(a.gU, a.vel_bytes);
+        float f[2 * NZ];
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          f[k] = gu_[k];
+          f[NZ + k] = gv_[k];
+        }
+#pragma unroll
+        for (int i = 0; i < NZ / 2; ++i) {
+          const v4u32 x = {__float_as_uint(f[4 * i]), __float_as_uint(f[4 * i + 1]),
+                           __float_as_uint(f[4 * i + 2]), __float_as_uint(f[4 * i + 3])};
+          __builtin_amdgcn_raw_buffer_store_b128(x, rG, on, 16 * i, 0);
+        }
+      }
 
       if constexpr (kDeferAdam) {
         // keep the updates after the sweep (the scheduler would hoist them
@@ -411,6 +455,13 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
   if (ks == 1 && cs == 2 * nz && rs == (int64_t)Nx * 2 * nz && V == U + nz) inter = 1;
   IGM_CHECK_ARG(inter >= 0, "unsupported velocity layout (need planar (Nz,Ny,Nx) or interleaved (Ny,Nx,2Nz))");
   IGM_CHECK_ARG(mode != 2 || inter == 0, "mode 2 needs the planar velocity layout");
+  // interleaved nodes as 16-B accesses: float32, even Nz (a node is a whole
+  // number of 16-B chunks), Y (and gY, with gV = gU + Nz) 16-B aligned and a
+  // batch stride of whole 16-B chunks
+  if (inter == 1 && sizeof(T) == 4 && nz % 2 == 0 && (reinterpret_cast<uintptr_t>(U) & 15) == 0 &&
+      (nbatch == 1 || (bs * (int64_t)sizeof(T)) % 16 == 0) &&
+      (mode != 1 || ((reinterpret_cast<uintptr_t>(gU) & 15) == 0 && gV == gU + nz)))
+    inter = 2;
   const int64_t per = inter ? plane * 2 * nz : plane * nz;  // elements of one batch entry
   if (nbatch > 1) IGM_CHECK_ARG(bs >= per, "batch stride smaller than one entry");
   const int64_t span = (int64_t)(nbatch - 1) * (nbatch > 1 ? bs : 0) + per;
@@ -454,6 +505,9 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
   if (mode == M && inter == I) launch_pass_nz<T, M, I>(nz, grid, st, a, &ok);
   IGM_MODE_LAYOUT(0, 0) IGM_MODE_LAYOUT(1, 0) IGM_MODE_LAYOUT(2, 0)
   IGM_MODE_LAYOUT(0, 1) IGM_MODE_LAYOUT(1, 1)
+  if constexpr (sizeof(T) == 4) {
+    IGM_MODE_LAYOUT(0, 2) IGM_MODE_LAYOUT(1, 2)
+  }
 #undef IGM_MODE_LAYOUT
   IGM_CHECK_ARG(ok, "nz not compiled (supported: 2,3,4,5,6,8,10,12,16,20)");
   IGM_CHECK_LAUNCH();

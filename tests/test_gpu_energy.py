@@ -103,6 +103,28 @@ def test_layout_nhwc_equals_planar(dev):
     assert torch.equal(gY[0, ..., nz:].permute(2, 0, 1), gV)
 
 
+def test_layout_nhwc_unaligned_equals_aligned(dev):
+    """A Y view that is not 16-byte aligned takes the one-level-per-access
+    path; a 16-byte aligned Y moves whole nodes as 16-B accesses.  Same
+    arithmetic, so the same energies and gradient bit for bit (Nz 10 and the
+    odd Nz 5, which always takes the one-level path)."""
+    for nz in (10, 5):
+        ny, nx = 37, 70
+        g = small_glacier(ny, nx, seed=nz)
+        U, V = smooth_velocity(nz, ny, nx, seed=nz + 1)
+        op = gpu_op(g, nz, dev, torch.float32)
+        Y = torch.cat([to_dev(U, dev).permute(1, 2, 0), to_dev(V, dev).permute(1, 2, 0)], -1)[None].contiguous()
+        assert Y.data_ptr() % 16 == 0
+        buf = torch.empty(Y.numel() + 1, dtype=torch.float32, device=dev)
+        Yu = buf[1:].view(Y.shape)
+        Yu.copy_(Y)
+        assert Yu.data_ptr() % 16 == 4
+        e1, g1 = op.energy_grad(Y)
+        e2, g2 = op.energy_grad(Yu)
+        assert torch.equal(e1, e2), nz
+        assert torch.equal(g1, g2), nz
+
+
 def test_batch_with_extents_equals_single(dev):
     """Patches of uneven sizes packed in one padded batch: masking by extents."""
     nz = 10
