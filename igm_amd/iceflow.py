@@ -207,7 +207,7 @@ def refine_emulated(cfg, state):
                     lr_schedule=slv.adam_schedule(lr, n_flat, n - n_flat, float(rf["lr_end"])))
     out = velocity_outputs(res.U, res.V, state.thk, state.vert_weight, int(ice.numerics.Nz))
     _set_outputs(cfg, state, out)
-    state.COST_REFINED = float(res.costs[-1])
+    state.COST_REFINED = res.costs_device[-1]  # float64 0-dim device tensor (no host wait)
 
 
 # ----------------------------------------------------------- fallback --
@@ -220,8 +220,8 @@ def fallback_check(cfg, state):
     from the emulated field.  If J_emu > J_check + rel_energy_gap * |J_check|,
     the solve continues (up to ``max_its`` iterations with early stopping) and
     its field replaces the emulated one.  With check_its = 0 the check only
-    records J_emu.  Records state.COST_EMULATED, state.COST_CHECK and
-    state.fallback_count.
+    records J_emu.  Records state.COST_EMULATED (a float64 0-dim device
+    tensor), state.COST_CHECK (float) and state.fallback_count.
     """
     fb = cfg.processes.iceflow.emulator.fallback
     state.fallback_calls = getattr(state, "fallback_calls", 0) + 1
@@ -229,11 +229,12 @@ def fallback_check(cfg, state):
     if (state.fallback_calls - 1) % max(1, int(fb.check_freq)) != 0:
         return
     op = energy_operator(cfg, state)
-    j_emu = float(op.energy(state.U, state.V)[0, 3])
-    state.COST_EMULATED = j_emu
+    j_emu_dev = op.energy(state.U, state.V)[0, 3]
+    state.COST_EMULATED = j_emu_dev  # float64 0-dim device tensor
     k = int(fb.check_its)
     if k <= 0:
-        return
+        return  # (nothing to decide: the host does not wait for the GPU)
+    j_emu = float(j_emu_dev)
     res = slv.solve(op, state.U, state.V, k, state.optimizer, early_stopping=False)
     j_chk = float(min(np.min(res.costs), j_emu))
     state.COST_CHECK = j_chk

@@ -52,8 +52,21 @@ class LegacyAdamState:
 
 
 class SolveResult:
+    """U, V (device), the COST of every iteration, the iteration count and
+    whether EarlyStopping ended the solve.  ``costs_device`` is the float64
+    COST history on the device; ``costs`` reads it to a numpy array on first
+    use (so a caller that does not look at it never waits for the GPU)."""
+
     def __init__(self, U, V, costs, iterations, stopped):
-        self.U, self.V, self.costs, self.iterations, self.stopped = U, V, costs, iterations, stopped
+        self.U, self.V, self.iterations, self.stopped = U, V, iterations, stopped
+        self.costs_device = costs if isinstance(costs, torch.Tensor) else None
+        self._costs = None if isinstance(costs, torch.Tensor) else costs
+
+    @property
+    def costs(self):
+        if self._costs is None:
+            self._costs = self.costs_device.cpu().numpy()
+        return self._costs
 
 
 def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early_stopping=True,
@@ -88,7 +101,10 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     # own partials to P[i % 2].  The state of the last launched pass is
     # E[n % 2], where a separate finalize then accounts for iteration n - 1.
     es_size = C.sizeof(L.IgmEarlyStop)
-    es_dev = torch.frombuffer(bytearray(bytes(es_host) * 2), dtype=torch.uint8).to(dev)
+    # (host-to-device copies from pinned memory are queued on the stream and
+    # do not make the host wait for the GPU work before them)
+    es_dev = torch.frombuffer(bytearray(bytes(es_host) * 2), dtype=torch.uint8).pin_memory().to(
+        dev, non_blocking=True)
     E = [es_dev.data_ptr(), es_dev.data_ptr() + es_size]
     P = [op.partials, torch.zeros_like(op.partials)]
     hist = torch.zeros(max(1, nbitmax), dtype=torch.float64, device=dev)
@@ -114,7 +130,8 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
         # launching once the solve has ended.  A last launch accounts the
         # final pass; the field is then in buffer passes % 2.
         M = [mom, torch.empty_like(mom)]
-        atab = torch.from_numpy(np.ascontiguousarray(alphas, dtype=np.float32)).to(dev)
+        atab = torch.from_numpy(np.ascontiguousarray(alphas, dtype=np.float32)).pin_memory().to(
+            dev, non_blocking=True)
 
         def launch(k, finalize_only=False):
             op.run_solve_pass(Ub[k % 2], Vb[k % 2], Ub[(k + 1) % 2], Vb[(k + 1) % 2], M[k % 2],
@@ -124,9 +141,15 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
 
         k = 0
         if not early_stopping:
-            for k in range((nbitmax + 1) // 2):
-                launch(k)
+            # every launch runs one or two iterations and nothing stops the
+            # solve: nbitmax iterations in k passes, the field in buffer
+            # k % 2; the host does not wait for the GPU
             k = (nbitmax + 1) // 2
+            for j in range(k):
+                launch(j)
+            launch(k, finalize_only=True)
+            optimizer.iterations = t0 + nbitmax
+            return SolveResult(Ub[k % 2], Vb[k % 2], hist[:nbitmax], nbitmax, False)
         else:
             while k < nbitmax:
                 launch(k)
@@ -140,7 +163,7 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
         n = int(st.iters_done)
         optimizer.iterations = t0 + n
         r = int(st.passes) % 2
-        return SolveResult(Ub[r], Vb[r], hist[:n].cpu().numpy(), n, bool(st.stopped))
+        return SolveResult(Ub[r], Vb[r], hist[:n], n, bool(st.stopped))
 
     n_launched = 0
     for i in range(nbitmax):
