@@ -261,8 +261,8 @@ def main():
         """Shortest schedule of the family {Adam (beta1, beta2) in BETAS;
         lr0 in LR0S; a fraction in FLATS at lr0, then a geometric decay to a
         step in ENDS} from (U0, V0) that ends <= 1 %: the first total of
-        ``totals`` that does, then bisected (4 steps, to 10 iterations)
-        against the total before it.  Returns a dict of its, err, lr0, flat,
+        ``totals`` that does, then bisected (to 2 iterations, one
+        two-iteration pass) against the total before it.  Returns a dict of its, err, lr0, flat,
         lr_end, beta1, beta2 (the last total tried if none reaches 1 %)."""
 
         def trial(total):
@@ -294,8 +294,8 @@ def main():
         if best["err"] > 0.01 or lo is None:
             return best
         hi = best["its"]
-        for _ in range(4):
-            mid = int(round((lo + hi) / 20.0)) * 10
+        for _ in range(8):
+            mid = int(round((lo + hi) / 4.0)) * 2
             if mid <= lo or mid >= hi:
                 break
             t = trial(mid)
