@@ -70,6 +70,10 @@ _SIGS = {
     "igm_conv_wgrad_bf16": (C.c_int, [I32, VP, VP, VP, I32, I32, I32, I32, I32, VP, I32, VP, VP, VP]),
     "igm_conv_in_wgrad": (C.c_int, [VP, VP, VP, VP, VP, I64, I64, I64, VP, I32, I32, I32, VP, I32, VP, I32,
                                     VP, VP, VP]),
+    "igm_conv_wgrad_partials_bf16": (C.c_int, [I32, VP, VP, VP, I32, I32, I32, I32, VP, I32, VP]),
+    "igm_conv_in_wgrad_partials": (C.c_int, [VP, VP, VP, VP, VP, I64, I64, I64, VP, I32, I32, I32, VP,
+                                             I32, VP, I32, VP]),
+    "igm_conv_wgrad_reduce_f32": (C.c_int, [I32, I32, I32, VP, I32, VP, VP, VP]),
     "igm_conv_in_dgrad": (C.c_int, [VP, VP, VP, I32, I32, I32, VP, VP]),
     "igm_vert_flow_kinematic_f32": (C.c_int, [I32, I32, I32, VP, VP, VP, VP, VP, VP, C.c_double, VP, VP, VP,
                                               VP]),

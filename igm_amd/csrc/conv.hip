@@ -514,7 +514,8 @@ constexpr int kWgRows = 4;  // dZ rows per tile (8 measured slower on c4 batches
 template <int TAPS>
 __global__ void __launch_bounds__(256)
 wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
-                  const int32_t* __restrict__ ext, int B, int H, int W, float* __restrict__ partials) {
+                  const int32_t* __restrict__ ext, int B, int H, int W, int accumulate,
+                  float* __restrict__ partials) {
   constexpr int AROWS = TAPS == 9 ? kWgRows + 2 : kWgRows;  // activation rows in LDS
   constexpr int NT = TAPS;  // accumulators per wave
   __shared__ __attribute__((aligned(16))) uint32_t lds_dz[kWgRows][kC][kChStride / 2];
@@ -529,6 +530,18 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
   f32x4 acc[NT];
 #pragma unroll
   for (int i = 0; i < NT; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // accumulate: this block's partial sums of the previous sub-batches, loaded
+  // now so that their latency hides behind the tiles (added at the end)
+  float* P = partials + (int64_t)blockIdx.x * (TAPS * kC * kC + kC);
+  const int ci = cib * 16 + (lane & 15);
+  f32x4 prev[NT];
+#pragma unroll
+  for (int tap = 0; tap < NT; ++tap)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int co = cb * 16 + 4 * (lane >> 4) + e;
+      prev[tap][e] = accumulate ? P[(tap * kC + co) * kC + ci] : 0.f;
+    }
   float dba[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) dba[e] = 0.f;
@@ -678,14 +691,12 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
     }
   }
   // partial dW: D[co][ci], lane: ci = cib*16 + lane%16, co = cb*16 + 4*(lane/16) + e
-  float* P = partials + (int64_t)blockIdx.x * (TAPS * kC * kC + kC);
-  const int ci = cib * 16 + (lane & 15);
 #pragma unroll
   for (int tap = 0; tap < NT; ++tap)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int co = cb * 16 + 4 * (lane >> 4) + e;
-      P[(tap * kC + co) * kC + ci] = acc[tap][e];
+      P[(tap * kC + co) * kC + ci] = accumulate ? prev[tap][e] + acc[tap][e] : acc[tap][e];
     }
   // partial db: thread t holds channels (t&3)*8 .. +8
 #pragma unroll
@@ -695,7 +706,7 @@ wgrad_mfma_kernel(const bf16* __restrict__ act, const bf16* __restrict__ dz,
     const int chunk = tid >> 3, e = tid & 7;
     float s = 0.f;
     for (int t = chunk; t < 256; t += 4) s += dbred[t][e];
-    P[TAPS * kC * kC + tid] = s;
+    P[TAPS * kC * kC + tid] = accumulate ? P[TAPS * kC * kC + tid] + s : s;
   }
 }
 
@@ -777,7 +788,7 @@ conv_in_wgrad_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
                      const float* __restrict__ f2, const float* __restrict__ f3,
                      const float* __restrict__ f4, int64_t fb, int64_t fr, int64_t fc,
                      const int32_t* __restrict__ ext, int B, int H, int W,
-                     const bf16* __restrict__ dz, float* __restrict__ partials) {
+                     const bf16* __restrict__ dz, int accumulate, float* __restrict__ partials) {
   __shared__ float xs[kWgTileH + 2][kWgTileW + 2][kCin];
   __shared__ float4 dzs[kWgTileH][kWgTileW][kC / 4];
   __shared__ float red[4][kWgOut];
@@ -859,7 +870,21 @@ conv_in_wgrad_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
   }
   __syncthreads();
   float* P = partials + (int64_t)blockIdx.x * kWgOut;
-  for (int i = tid; i < kWgOut; i += 256) P[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+  constexpr int kPer = (kWgOut + 255) / 256;
+  float prev[kPer];  // (all loads before the first store)
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = tid + 256 * k;
+    prev[k] = (accumulate && i < kWgOut) ? P[i] : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = tid + 256 * k;
+    if (i < kWgOut) {
+      const float v = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+      P[i] = accumulate ? prev[k] + v : v;
+    }
+  }
 }
 
 // Stage 2 as a plain column sum: out[i] = sum over the group rows of P[.][i],
@@ -1033,6 +1058,75 @@ int64_t igm_wgrad_workspace_floats(int32_t taps, int32_t nblocks) {
   return 0;
 }
 
+int igm_conv_wgrad_partials_bf16(int32_t taps, const void* act, const void* dz,
+                                 const int32_t* extents, int32_t B, int32_t H, int32_t W,
+                                 int32_t nblocks, float* partials, int32_t accumulate,
+                                 void* stream) {
+  IGM_CHECK_ARG(act && dz && partials, "null pointer");
+  IGM_CHECK_ARG(taps == 9 || taps == 1, "taps must be 9 or 1");
+  IGM_CHECK_ARG(nblocks >= 1 && B >= 1 && H >= 1 && W >= 1, "bad sizes");
+  IGM_CHECK_ARG((int64_t)B * H * W * kC * 2 < (int64_t)kOOB, "activations above 2 GiB (32-bit buffer offsets)");
+  hipStream_t st = (hipStream_t)stream;
+  const int acc = accumulate ? 1 : 0;
+  if (taps == 9)
+    hipLaunchKernelGGL(wgrad_mfma_kernel<9>, dim3(nblocks), dim3(256), 0, st, (const bf16*)act,
+                       (const bf16*)dz, extents, B, H, W, acc, partials);
+  else
+    hipLaunchKernelGGL(wgrad_mfma_kernel<1>, dim3(nblocks), dim3(256), 0, st, (const bf16*)act,
+                       (const bf16*)dz, extents, B, H, W, acc, partials);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_conv_in_wgrad_partials(const float* f0, const float* f1, const float* f2, const float* f3,
+                               const float* f4, int64_t fb, int64_t fr, int64_t fc,
+                               const int32_t* extents, int32_t B, int32_t H, int32_t W,
+                               const void* dz1, int32_t nblocks, float* partials,
+                               int32_t accumulate, void* stream) {
+  IGM_CHECK_ARG(f0 && f1 && f2 && f3 && f4 && dz1 && partials, "null pointer");
+  IGM_CHECK_ARG(nblocks >= 1 && B >= 1 && H >= 1 && W >= 1, "bad sizes");
+  IGM_CHECK_ARG((int64_t)B * H * W * kC * 2 < (int64_t)kOOB, "activations above 2 GiB (32-bit buffer offsets)");
+  hipLaunchKernelGGL(conv_in_wgrad_kernel, dim3(nblocks), dim3(256), 0, (hipStream_t)stream, f0,
+                     f1, f2, f3, f4, fb, fr, fc, extents, B, H, W, (const bf16*)dz1,
+                     accumulate ? 1 : 0, partials);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
+int igm_conv_wgrad_reduce_f32(int32_t taps, int32_t cout, int32_t nblocks, float* partials,
+                              int32_t accumulate, float* dW_hwio, float* db, void* stream) {
+  IGM_CHECK_ARG(partials && dW_hwio && db, "null pointer");
+  IGM_CHECK_ARG(taps == 9 || taps == 1 || taps == 0, "taps must be 9, 1 or 0 (layer 1)");
+  IGM_CHECK_ARG(cout >= 1 && cout <= kC, "cout must be in [1, 32]");
+  IGM_CHECK_ARG(taps != 0 || cout == kC, "layer 1 has 32 outputs");
+  IGM_CHECK_ARG(nblocks >= 1, "bad sizes");
+  hipStream_t st = (hipStream_t)stream;
+  int R, G;
+  group_rows(nblocks, &R, &G);
+  if (taps == 0) {
+    // the layer-1 partial layout is [blk][1440 + 32] = [blk][n]: a plain column sum
+    const int n = kWgOut;
+    hipLaunchKernelGGL(partials_group_sum_kernel, dim3((n + 255) / 256, G), dim3(256), 0, st,
+                       partials, nblocks, n, n, R);
+    IGM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(sum_partials_kernel, dim3((9 * kCin * kC + 255) / 256), dim3(256), 0, st,
+                       partials, G, R, n, 9 * kCin * kC, accumulate, dW_hwio);
+    IGM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partials + 9 * kCin * kC,
+                       G, R, n, kC, accumulate, db);
+    IGM_CHECK_LAUNCH();
+    return 0;
+  }
+  const int entries = taps * kC * kC + kC;
+  hipLaunchKernelGGL(partials_group_sum_kernel, dim3((entries + 255) / 256, G), dim3(256), 0, st,
+                     partials, nblocks, entries, entries, R);
+  IGM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((entries + 255) / 256), dim3(256), 0, st, partials,
+                     G, R, taps, cout, accumulate, dW_hwio, db);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
+
 int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int32_t* extents,
                         int32_t B, int32_t H, int32_t W, int32_t cout, int32_t nblocks,
                         float* workspace, int32_t accumulate, float* dW_hwio, float* db,
@@ -1040,26 +1134,10 @@ int igm_conv_wgrad_bf16(int32_t taps, const void* act, const void* dz, const int
   IGM_CHECK_ARG(act && dz && workspace && dW_hwio && db, "null pointer");
   IGM_CHECK_ARG(taps == 9 || taps == 1, "taps must be 9 or 1");
   IGM_CHECK_ARG(cout >= 1 && cout <= kC, "cout must be in [1, 32]");
-  IGM_CHECK_ARG(nblocks >= 1 && B >= 1 && H >= 1 && W >= 1, "bad sizes");
-  IGM_CHECK_ARG((int64_t)B * H * W * kC * 2 < (int64_t)kOOB, "activations above 2 GiB (32-bit buffer offsets)");
-  hipStream_t st = (hipStream_t)stream;
-  if (taps == 9)
-    hipLaunchKernelGGL(wgrad_mfma_kernel<9>, dim3(nblocks), dim3(256), 0, st, (const bf16*)act,
-                       (const bf16*)dz, extents, B, H, W, workspace);
-  else
-    hipLaunchKernelGGL(wgrad_mfma_kernel<1>, dim3(nblocks), dim3(256), 0, st, (const bf16*)act,
-                       (const bf16*)dz, extents, B, H, W, workspace);
-  IGM_CHECK_LAUNCH();
-  const int entries = taps * kC * kC + kC;
-  int R, G;
-  group_rows(nblocks, &R, &G);
-  hipLaunchKernelGGL(partials_group_sum_kernel, dim3((entries + 255) / 256, G), dim3(256), 0, st,
-                     workspace, nblocks, entries, entries, R);
-  IGM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((entries + 255) / 256), dim3(256), 0, st, workspace,
-                     G, R, taps, cout, accumulate, dW_hwio, db);
-  IGM_CHECK_LAUNCH();
-  return 0;
+  if (const int rc = igm_conv_wgrad_partials_bf16(taps, act, dz, extents, B, H, W, nblocks, workspace,
+                                                  0, stream))
+    return rc;
+  return igm_conv_wgrad_reduce_f32(taps, cout, nblocks, workspace, accumulate, dW_hwio, db, stream);
 }
 
 int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const float* f3,
@@ -1068,26 +1146,10 @@ int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const f
                       float* workspace, int32_t accumulate, float* dW_hwio, float* db,
                       void* stream) {
   IGM_CHECK_ARG(f0 && f1 && f2 && f3 && f4 && dz1 && workspace && dW_hwio && db, "null pointer");
-  IGM_CHECK_ARG(nblocks >= 1 && B >= 1 && H >= 1 && W >= 1, "bad sizes");
-  IGM_CHECK_ARG((int64_t)B * H * W * kC * 2 < (int64_t)kOOB, "activations above 2 GiB (32-bit buffer offsets)");
-  hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(conv_in_wgrad_kernel, dim3(nblocks), dim3(256), 0, st, f0, f1, f2, f3, f4, fb,
-                     fr, fc, extents, B, H, W, (const bf16*)dz1, workspace);
-  IGM_CHECK_LAUNCH();
-  const int n = 9 * kCin * kC + kC;
-  // the layer-1 partial layout is [blk][1440 + 32] = [blk][n]: a plain column sum
-  int R, G;
-  group_rows(nblocks, &R, &G);
-  hipLaunchKernelGGL(partials_group_sum_kernel, dim3((n + 255) / 256, G), dim3(256), 0, st,
-                     workspace, nblocks, n, n, R);
-  IGM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(sum_partials_kernel, dim3((9 * kCin * kC + 255) / 256), dim3(256), 0, st,
-                     workspace, G, R, n, 9 * kCin * kC, accumulate, dW_hwio);
-  IGM_CHECK_LAUNCH();
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, workspace + 9 * kCin * kC,
-                     G, R, n, kC, accumulate, db);
-  IGM_CHECK_LAUNCH();
-  return 0;
+  if (const int rc = igm_conv_in_wgrad_partials(f0, f1, f2, f3, f4, fb, fr, fc, extents, B, H, W,
+                                                dz1, nblocks, workspace, 0, stream))
+    return rc;
+  return igm_conv_wgrad_reduce_f32(0, kC, nblocks, workspace, accumulate, dW_hwio, db, stream);
 }
 
 int igm_conv_in_dgrad(const void* dz1, const float* w_hwio, const int32_t* extents, int32_t B,

@@ -134,39 +134,87 @@ class CNN:
         return (Y, acts) if keep else Y
 
     # ----------------------------------------------------------- backward --
-    def backward(self, dY, acts, fields, extents=None, nblocks=512, accumulate=False, input_grad=None):
+    def backward(self, dY, acts, fields, extents=None, nblocks=512, accumulate=False, input_grad=None,
+                 deferred=False):
         """Gradient of sum(dY * Y) w.r.t. all parameters, into self.g (added
         to it when ``accumulate``).  ``input_grad`` (optional, fp32 (B, H, W,
         5)) receives the gradient w.r.t. the input fields, the quantity data
-        assimilation differentiates through the emulator."""
+        assimilation differentiates through the emulator.
+
+        ``deferred``: for one batch split into sub-batches, each sub-batch's
+        weight gradients stay as per-block partial sums in persistent
+        per-layer buffers (``accumulate``: added to the previous
+        sub-batches'), and ``reduce_deferred()`` sums them into self.g once:
+        one reduction per layer per batch instead of one per sub-batch."""
         acc = 1 if accumulate else 0
         fptrs, (fb, fr, fc), B, H, W = fields
         ext = L.ptr(extents)
         st = L.stream_ptr()
         lib = L.load()
-        ws_n = max(lib.igm_wgrad_workspace_floats(9, nblocks), lib.igm_wgrad_workspace_floats(0, nblocks))
-        ws = torch.empty(ws_n, dtype=torch.float32, device=self.device)
+        if deferred:
+            part = self._deferred_partials(nblocks)
+
+            def wgrad(li, taps, a, d, cout):
+                L.call("igm_conv_wgrad_partials_bf16", taps, L.ptr(a), L.ptr(d), ext, B, H, W, nblocks,
+                       L.ptr(part[li]), acc, st)
+
+            def wgrad_in(d):
+                L.call("igm_conv_in_wgrad_partials", *fptrs, fb, fr, fc, ext, B, H, W, L.ptr(d), nblocks,
+                       L.ptr(part[0]), acc, st)
+        else:
+            ws_n = max(lib.igm_wgrad_workspace_floats(9, nblocks), lib.igm_wgrad_workspace_floats(0, nblocks))
+            ws = torch.empty(ws_n, dtype=torch.float32, device=self.device)
+
+            def wgrad(li, taps, a, d, cout):
+                L.call("igm_conv_wgrad_bf16", taps, L.ptr(a), L.ptr(d), ext, B, H, W, cout, nblocks,
+                       L.ptr(ws), acc, L.ptr(self.kernel(li, self.g)), L.ptr(self.bias(li, self.g)), st)
+
+            def wgrad_in(d):
+                L.call("igm_conv_in_wgrad", *fptrs, fb, fr, fc, ext, B, H, W, L.ptr(d), nblocks, L.ptr(ws),
+                       acc, L.ptr(self.kernel(0, self.g)), L.ptr(self.bias(0, self.g)), st)
         nl = self.nl
         dz = torch.empty((B, H, W, NF), dtype=torch.bfloat16, device=self.device)
         dYb = torch.empty_like(dz)
         L.call("igm_conv1x1_head_bwd", L.ptr(dY), L.ptr(acts[nl - 1]), L.ptr(self.kernel(nl)), ext, B, H,
                W, self.cout, L.ptr(dz), L.ptr(dYb), st)
-        L.call("igm_conv_wgrad_bf16", 1, L.ptr(acts[nl - 1]), L.ptr(dYb), ext, B, H, W, self.cout,
-               nblocks, L.ptr(ws), acc, L.ptr(self.kernel(nl, self.g)), L.ptr(self.bias(nl, self.g)), st)
+        wgrad(nl, 1, acts[nl - 1], dYb, self.cout)
         below = dYb  # reuse as the ping-pong buffer
         for li in range(nl - 1, 0, -1):  # layer index li (0-based) is a 3x3 MFMA layer
             a_in = acts[li - 1]
-            L.call("igm_conv_wgrad_bf16", 9, L.ptr(a_in), L.ptr(dz), ext, B, H, W, NF, nblocks,
-                   L.ptr(ws), acc, L.ptr(self.kernel(li, self.g)), L.ptr(self.bias(li, self.g)), st)
+            wgrad(li, 9, a_in, dz, NF)
             L.call("igm_conv3x3_dgrad_bf16", L.ptr(dz), L.ptr(self.dgr[li - 1]), L.ptr(a_in), ext, B, H,
                    W, L.ptr(below), st)
             dz, below = below, dz
-        L.call("igm_conv_in_wgrad", *fptrs, fb, fr, fc, ext, B, H, W, L.ptr(dz), nblocks, L.ptr(ws), acc,
-               L.ptr(self.kernel(0, self.g)), L.ptr(self.bias(0, self.g)), st)
+        wgrad_in(dz)
         if input_grad is not None:
             assert input_grad.shape == (B, H, W, 5) and input_grad.dtype == torch.float32
             assert input_grad.is_contiguous()
             L.call("igm_conv_in_dgrad", L.ptr(dz), L.ptr(self.kernel(0)), ext, B, H, W, L.ptr(input_grad), st)
+        return self.g
+
+    def _deferred_partials(self, nblocks):
+        """Persistent per-layer partial-sum buffers of the deferred backward
+        (layer 0: layer 1's layout; 1..nl-1: 3x3; nl: the 1x1 head)."""
+        if getattr(self, "_part_nblocks", None) != nblocks:
+            lib = L.load()
+            n = {0: lib.igm_wgrad_workspace_floats(0, nblocks), 9: lib.igm_wgrad_workspace_floats(9, nblocks),
+                 1: lib.igm_wgrad_workspace_floats(1, nblocks)}
+            self._part = [torch.empty(n[0 if li == 0 else (1 if li == self.nl else 9)], dtype=torch.float32,
+                                      device=self.device) for li in range(self.nl + 1)]
+            self._part_nblocks = nblocks
+        return self._part
+
+    def reduce_deferred(self, accumulate=False):
+        """Sum the deferred partials of every layer into self.g (added to it
+        when ``accumulate``), one fixed-order reduction per layer; the
+        partials are consumed."""
+        nb = self._part_nblocks
+        st = L.stream_ptr()
+        for li in range(self.nl + 1):
+            taps = 0 if li == 0 else (1 if li == self.nl else 9)
+            cout = self.cout if li == self.nl else NF
+            L.call("igm_conv_wgrad_reduce_f32", taps, cout, nb, L.ptr(self._part[li]), 1 if accumulate else 0,
+                   L.ptr(self.kernel(li, self.g)), L.ptr(self.bias(li, self.g)), st)
         return self.g
 
     # ---------------------------------------------------------- optimizer --

@@ -197,8 +197,9 @@ def train(cnn, batcher, dzeta, phys, steps=None, seconds=None, lr=1e-4, lr_end=N
             fields = EM.nhwc_fields(X)
             Y, acts = cnn.forward(fields, extents=ext, keep=True)
             e, gY = op.energy_grad(Y)
-            cnn.backward(gY, acts, fields, extents=ext, accumulate=k > 0)
+            cnn.backward(gY, acts, fields, extents=ext, accumulate=k > 0, deferred=True)
             costs.append(e[:, 3])
+        cnn.reduce_deferred()
         lr_s = lr if lr_end is None else lr * (lr_end / lr) ** min(1.0, frac)
         cnn.adam_step(lr_s, clipnorm=clipnorm, epsilon=epsilon)
         if avg is not None:

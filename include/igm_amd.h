@@ -280,6 +280,28 @@ int igm_conv_in_wgrad(const float* f0, const float* f1, const float* f2, const f
                       float* workspace, int32_t accumulate, float* dW_hwio, float* db,
                       void* stream);
 
+/* The two halves of igm_conv_wgrad_bf16 / igm_conv_in_wgrad, for a batch
+ * split into sub-batches (the gradient of one training batch,
+ * pretraining/pretraining.py:198-493, over patches of uneven sizes padded per
+ * sub-batch): the *_partials calls write (accumulate == 0) or add to
+ * (accumulate != 0) the per-block partial sums in `partials`
+ * (igm_wgrad_workspace_floats(taps, nblocks) floats, the same nblocks on every
+ * call); igm_conv_wgrad_reduce_f32 (taps 9, 1, or 0 for layer 1 with cout 32)
+ * then sums them into dW_hwio, db once, in a fixed order, and consumes
+ * `partials` (its group sums overwrite rows).  Sub-batch k adds to the
+ * partials of sub-batches 0..k-1 in stream order: deterministic. */
+int igm_conv_wgrad_partials_bf16(int32_t taps, const void* act, const void* dz,
+                                 const int32_t* extents, int32_t B, int32_t H, int32_t W,
+                                 int32_t nblocks, float* partials, int32_t accumulate,
+                                 void* stream);
+int igm_conv_in_wgrad_partials(const float* f0, const float* f1, const float* f2, const float* f3,
+                               const float* f4, int64_t fb, int64_t fr, int64_t fc,
+                               const int32_t* extents, int32_t B, int32_t H, int32_t W,
+                               const void* dz1, int32_t nblocks, float* partials,
+                               int32_t accumulate, void* stream);
+int igm_conv_wgrad_reduce_f32(int32_t taps, int32_t cout, int32_t nblocks, float* partials,
+                              int32_t accumulate, float* dW_hwio, float* db, void* stream);
+
 /* Gradient of the emulator with respect to its 5 input fields (the tape
  * gradient through the network of data_assimilation/optimize/update.py:29-73),
  * through layer 1: dX (fp32 NHWC [B][H][W][5]) from dz1, the bf16 gradient at

@@ -302,7 +302,7 @@ def test_bucketed_gradient_equals_padded(dev):
         ext = torch.tensor([list(p.shape[:2]) for p in ps], dtype=torch.int32, device=dev)
         return X, ext
 
-    def grad(groups):
+    def grad(groups, deferred=False):
         keep = []
         for k, ps in enumerate(groups):
             X, ext = pack(ps)
@@ -310,8 +310,10 @@ def test_bucketed_gradient_equals_padded(dev):
             fields = EM.nhwc_fields(X)
             Y, acts = cnn.forward(fields, extents=ext, keep=True)
             e, gY = op.energy_grad(Y)
-            cnn.backward(gY, acts, fields, extents=ext, accumulate=k > 0)
+            cnn.backward(gY, acts, fields, extents=ext, accumulate=k > 0, deferred=deferred)
             keep.append((X, ext, fields, Y, acts, gY))
+        if deferred:
+            cnn.reduce_deferred()
         torch.cuda.synchronize()
         return cnn.g.cpu().clone()
 
@@ -319,3 +321,11 @@ def test_bucketed_gradient_equals_padded(dev):
     g_bkt = grad([[parts[0], parts[3]], [parts[1], parts[2]]])
     assert float(g_all.abs().max()) > 0
     assert rel(g_bkt, g_all) <= 1e-5
+    # deferred: the sub-batches add to per-block partials, one reduction per
+    # layer; a second batch overwrites the consumed partials (same bits)
+    g_def = grad([[parts[0], parts[3]], [parts[1], parts[2]]], deferred=True)
+    assert rel(g_def, g_all) <= 1e-5
+    g_def2 = grad([[parts[0], parts[3]], [parts[1], parts[2]]], deferred=True)
+    assert torch.equal(g_def2, g_def)
+    g_one = grad([parts], deferred=True)  # one sub-batch: the plain backward's bits
+    assert torch.equal(g_one, g_all)
