@@ -76,6 +76,9 @@ class Batcher:
         self.buckets = max(1, min(int(buckets), batch))
         self.rng = np.random.default_rng(seed)
         self.pixels = self.padded = 0  # valid and padded pixels served so far
+        # the 8 flip / transpose variants of every patch, made once and kept
+        # contiguous (8x the pool's memory; a step then only copies)
+        self.variants = [[augment(p, c).contiguous() for c in range(8)] for p in pool]
 
     def _pack(self, parts):
         H = max(p.shape[0] for p in parts)
@@ -94,7 +97,7 @@ class Batcher:
         """List of (X (b, H, W, 5), extents (b, 2) int32) sub-batches."""
         idx = self.rng.integers(0, len(self.pool), self.batch)
         codes = self.rng.integers(0, 8, self.batch)
-        parts = [augment(self.pool[i], int(c)) for i, c in zip(idx, codes)]
+        parts = [self.variants[i][int(c)] for i, c in zip(idx, codes)]
         if self.buckets == 1:
             return [self._pack(parts)]
         parts.sort(key=lambda p: p.shape[0] * p.shape[1])
