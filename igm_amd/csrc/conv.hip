@@ -395,6 +395,18 @@ head_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
     wf[cb] = *reinterpret_cast<const bf16x8*>(wt + ((cb * 16 + (lane & 15)) * kC + 8 * (lane >> 4)));
   const bf16* ib = in + (int64_t)b * npx * kC;
   float* Yb = Y + (int64_t)b * npx * cout;
+  // this lane's output channels co = cb*16 + 4*(lane/16) + e: their biases,
+  // and whether all 4 are real (cout % 4 == 0 makes a pixel's row of cout
+  // floats 16-B aligned: one 16-B store per lane and channel block)
+  float bl[2][4];
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int co = cb * 16 + 4 * (lane >> 4) + e;
+      bl[cb][e] = co < cout ? bias[co] : 0.f;
+    }
+  const bool vec = (cout % 4) == 0 && (reinterpret_cast<uintptr_t>(Y) & 15) == 0;
 #pragma unroll
   for (int pb = 0; pb < 4; ++pb) {
     const int64_t p = p0 + pb * 16 + (lane & 15);
@@ -415,9 +427,16 @@ head_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ wt,
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
       const int co = cb * 16 + 4 * (lane >> 4);
+      float o[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (co + e < cout) Yb[p * cout + co + e] = valid ? acc[cb][e] + bias[co + e] : 0.f;
+      for (int e = 0; e < 4; ++e) o[e] = valid ? acc[cb][e] + bl[cb][e] : 0.f;
+      if (vec) {
+        if (co < cout) *reinterpret_cast<float4*>(Yb + p * cout + co) = make_float4(o[0], o[1], o[2], o[3]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (co + e < cout) Yb[p * cout + co + e] = o[e];
+      }
     }
   }
 }
