@@ -124,6 +124,42 @@ def test_conv_in_and_head(dev):
     assert rel(Y.cpu(), refY) <= 1e-5
 
 
+@pytest.mark.parametrize("cout,offset", [(20, 0), (20, 1), (18, 0), (7, 0), (32, 0), (32, 3)])
+def test_head_store_paths_and_extents(dev, cout, offset):
+    """The head writes 16-B stores when cout % 4 == 0 and Y is 16-B aligned,
+    4-B stores otherwise (odd cout, or Y offset by `offset` floats); pixels
+    outside a sample's extent are zero and channels beyond cout untouched."""
+    B, H, W = 3, 21, 45
+    g = torch.Generator().manual_seed(cout * 10 + offset)
+    a = bf(torch.randn((B, H, W, 32), generator=g))
+    wt = bf(torch.randn((32, 32), generator=g) * 0.2)
+    wt[cout:] = 0.0  # rows beyond cout are the zero padding of the packed layout
+    bias = torch.randn(cout, generator=g)
+    ext = torch.tensor([[H, W], [13, 30], [1, W]], dtype=torch.int32)
+    n = B * H * W * cout
+    buf = torch.full((n + offset + 8,), 7.0, dtype=torch.float32, device=dev)
+    Y = buf[offset:offset + n]
+    ad = a.to(torch.bfloat16).to(dev)
+    wd = wt.to(torch.bfloat16).to(dev)
+    bd = bias.to(dev)
+    ed = ext.to(dev)
+    L.call("igm_conv1x1_head_fwd", L.ptr(ad), L.ptr(wd), L.ptr(bd), L.ptr(ed), B, H, W, cout, L.ptr(Y),
+           L.stream_ptr())
+    torch.cuda.synchronize()
+    refY = (a.double() @ wt[:cout].double().T + bias.double()).reshape(B, H, W, cout)
+    for b in range(B):
+        ny, nx = int(ext[b, 0]), int(ext[b, 1])
+        keep = torch.zeros((H, W), dtype=torch.bool)
+        keep[:ny, :nx] = True
+        refY[b][~keep] = 0.0
+    out = buf.cpu()
+    assert rel(out[offset:offset + n].reshape(B, H, W, cout), refY) <= 1e-5
+    # outside the padded region nothing is written
+    assert torch.all(out[:offset] == 7.0) and torch.all(out[offset + n:] == 7.0)
+    # the zeroed pixels are exactly zero
+    assert torch.all(out[offset:offset + n].reshape(B, H, W, cout)[1, 13:] == 0.0)
+
+
 def test_conv_in_wgrad(dev):
     B, H, W = 2, 30, 300
     g = torch.Generator().manual_seed(5)
