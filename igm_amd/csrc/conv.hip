@@ -475,9 +475,23 @@ head_bwd_kernel(const float* __restrict__ dY, const bf16* __restrict__ a16,
     }
     return;
   }
+  // dY's row of cout floats: 16-B loads when cout % 4 == 0 (the row is then
+  // 16-B aligned in a 16-B aligned dY), 4-B loads otherwise
   float d[kC];
+  if ((cout % 4) == 0 && (reinterpret_cast<uintptr_t>(dY) & 15) == 0) {
 #pragma unroll
-  for (int c = 0; c < kC; ++c) d[c] = c < cout ? dY[q * cout + c] : 0.f;
+    for (int g = 0; g < kC / 4; ++g) {
+      const float4 t = 4 * g < cout ? *reinterpret_cast<const float4*>(dY + q * cout + 4 * g)
+                                    : make_float4(0.f, 0.f, 0.f, 0.f);
+      d[4 * g] = t.x;
+      d[4 * g + 1] = t.y;
+      d[4 * g + 2] = t.z;
+      d[4 * g + 3] = t.w;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < kC; ++c) d[c] = c < cout ? dY[q * cout + c] : 0.f;
+  }
   // s[ci] = sum over co = 0 .. cout-1, in order, of d[co] * w[ci][co]
   float s[kC];
 #pragma unroll
