@@ -87,13 +87,18 @@ ICEFLOW = {
             "weight_initialization": "glorot_uniform",
             "cnn3d_for_vertical": False,
         },
-        # build-specific energy-gap fallback to the plain solver
+        # build-specific energy-gap fallback to the plain solver: every
+        # check_freq update() calls, check_its solver iterations (the
+        # solver's own step_size) from the emulated field give J_check, the
+        # lowest energy they see; the solve continues up to max_its
+        # iterations (with EarlyStopping) and replaces the emulated field
+        # when J_emulated > J_check + rel_energy_gap * |J_check|
         "fallback": {
             "enabled": True,
-            "rel_energy_gap": 0.01,   # fall back when J_emulated > J_check + gap*|J_check|
-            "check_its": 0,           # solver iterations from the emulated field giving J_check
-            "check_freq": 1,          # check every check_freq update() calls
-            "max_its": 2000,          # solver iterations of a fallback solve
+            "rel_energy_gap": 0.01,
+            "check_its": 10,
+            "check_freq": 1,
+            "max_its": 2000,
         },
         # build-specific polish of every emulated field: `its` plain-solver
         # iterations warm-started from it, step lr for flat_frac of them,

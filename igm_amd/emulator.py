@@ -360,6 +360,12 @@ def split_into_patches(Ny, Nx, nbmax):
     return [(j * ly, (j + 1) * ly, i * lx, (i + 1) * lx) for i in range(sx) for j in range(sy)]
 
 
+def retrain_lr(lr, lr_decay, i):
+    """emulate.py:209-210: learning rate of patch i of a retraining epoch,
+    lr * lr_decay^(i / 1000) when lr_decay < 1."""
+    return lr * (lr_decay ** (i / 1000)) if lr_decay < 1 else lr
+
+
 def retrain_step(model, X, op, lr, clipnorm, epsilon, extents=None):
     """One energy-as-loss step on a batch X (B, H, W, 5): returns COST (sum
     over the batch entries), the global gradient norm is left in model.g."""
@@ -401,8 +407,8 @@ def update_iceflow_emulator(cfg, state, it, pertubate=False):
     for epoch in range(nbit):
         cost_emulator = 0.0
         for i, (p, op) in enumerate(zip(patches, ops)):
-            lr_i = lr * (em.lr_decay ** (i / 1000)) if em.lr_decay < 1 else lr
-            e = retrain_step(model, p, op, lr_i, opt["clipnorm"], opt["epsilon"])
+            e = retrain_step(model, p, op, retrain_lr(lr, float(em.lr_decay), i), opt["clipnorm"],
+                             opt["epsilon"])
             cost_emulator = cost_emulator + e[0, 3]
         state.COST_EMULATOR.append(float(cost_emulator))
         state.GRAD_EMULATOR.append(float(torch.linalg.vector_norm(model.g)))

@@ -4,10 +4,13 @@ The analogue of igm/processes/pretraining/pretraining.py:198-493
 (train_iceflow_emulator) with generated data instead of netCDF datasets:
 each step draws a batch of glacier patches of uneven sizes, packs them into
 one zero-padded NHWC batch with per-entry extents (the kernels mask
-everything outside an entry's extent), and takes one legacy-Adam step on
-the sum over the batch of the Blatter-Pattyn energy of the predicted field
-(pretraining.py:318-338: loss = energy, no labels).  Augmentation: random
-flips and transposition of each patch (pretraining.py:550-559).
+everything outside an entry's extent), and takes one plain legacy-Adam step
+(no clipnorm, pretraining.py:222-228) on the mean over the batch of the
+Blatter-Pattyn energy of the predicted field (pretraining.py:318-338: loss =
+energy, no labels; the reference's batches hold equal-size samples, where
+its reduce_mean over all cells is this mean of per-entry means).
+Augmentation: random flips and transposition of each patch
+(pretraining.py:550-559).
 
 CLI: ``python -m igm_amd.pretraining --steps 2000 --out <file>`` trains
 from IGM's shipped weights (or from scratch with --scratch) and saves a
@@ -105,6 +108,33 @@ class Batcher:
         return [self._pack(parts[c0:c1]) for c0, c1 in zip(cuts[:-1], cuts[1:]) if c1 > c0]
 
 
+class PatchBatchBatcher:
+    """Fresh batches: step s serves ``synthetic.patch_batch(s)`` -- 32 patches
+    of sides 128..512 seeded by 3 + s (numpy on the host) -- as one
+    zero-padded batch with per-entry extents (``buckets`` = 1: one launch
+    per layer over the whole batch, the padding masked) or split by area
+    into ``buckets`` padded sub-batches."""
+
+    def __init__(self, dev, batch=32, start=0, buckets=1):
+        self.dev, self.batch, self.step, self.buckets = dev, batch, start, max(1, int(buckets))
+        self.pixels = self.padded = 0
+
+    def next(self):
+        parts = [patch_tensor(p, self.dev) for p in syn.patch_batch(self.step, n=self.batch)]
+        self.step += 1
+        packer = Batcher.__new__(Batcher)
+        packer.dev, packer.pixels, packer.padded = self.dev, 0, 0
+        if self.buckets == 1:
+            out = [packer._pack(parts)]
+        else:
+            parts.sort(key=lambda p: p.shape[0] * p.shape[1])
+            cuts = np.linspace(0, len(parts), self.buckets + 1).round().astype(int)
+            out = [packer._pack(parts[c0:c1]) for c0, c1 in zip(cuts[:-1], cuts[1:]) if c1 > c0]
+        self.pixels += packer.pixels
+        self.padded += packer.padded
+        return out
+
+
 class CropBatcher:
     """Batches of ``batch`` square crops of side ``crop`` from whole fields
     (random field, offset, flips and transposition): equal-size entries, so
@@ -174,11 +204,13 @@ class WeightAverage:
         self.cnn.pack()
 
 
-def train(cnn, batcher, dzeta, phys, steps=None, seconds=None, lr=1e-4, lr_end=None, clipnorm=1.0,
+def train(cnn, batcher, dzeta, phys, steps=None, seconds=None, lr=1e-4, lr_end=None, clipnorm=None,
           epsilon=1e-7, log_every=100, val=None, on_best=None, log=print, ema=None):
     """Energy-as-loss training for ``steps`` steps or ``seconds`` of wall
     time (whichever is given); returns the list of (step, mean COST per
-    entry[, validation error]).
+    entry[, validation error]).  The loss of a step is the mean over the
+    batch's entries of their COST; ``clipnorm`` (per-tensor clip_by_norm,
+    as the emulator's online retraining uses) is off unless given.
 
     lr decays geometrically from ``lr`` to ``lr_end`` over the run (constant
     if None).  ``val(cnn)`` (optional) is evaluated every log_every steps;
@@ -195,16 +227,19 @@ def train(cnn, batcher, dzeta, phys, steps=None, seconds=None, lr=1e-4, lr_end=N
         if (steps is not None and s >= steps) or (seconds is not None and frac >= 1.0):
             break
         costs = []
-        for k, (X, ext) in enumerate(batcher.next()):
+        parts = batcher.next()
+        n_entries = sum(int(X.shape[0]) for X, _ in parts)
+        for k, (X, ext) in enumerate(parts):
             op = EnergyOperator(dzeta, phys, X=X, extents=ext)
             fields = EM.nhwc_fields(X)
             Y, acts = cnn.forward(fields, extents=ext, keep=True)
             e, gY = op.energy_grad(Y)
+            gY.mul_(1.0 / n_entries)  # d(mean over the batch of COST_b)/dY
             cnn.backward(gY, acts, fields, extents=ext, accumulate=k > 0, deferred=True)
             costs.append(e[:, 3])
         cnn.reduce_deferred()
         lr_s = lr if lr_end is None else lr * (lr_end / lr) ** min(1.0, frac)
-        cnn.adam_step(lr_s, clipnorm=clipnorm, epsilon=epsilon)
+        cnn.adam_step(lr_s, clipnorm=0.0 if clipnorm is None else clipnorm, epsilon=epsilon)
         if avg is not None:
             avg.update()
         s += 1

@@ -13,7 +13,8 @@ it follows.  It exists to check the HIP implementation in ``igm_amd``:
 
 The reference is TensorFlow 2.15 code and cannot be imported or run here
 (TensorFlow is absent).  Autograd of this restatement plays the role of the
-reference's ``tf.GradientTape``.  How each function is pinned is listed in
-the docstring of ``oracle.pins``; a function without an independent pin says
+reference's ``tf.GradientTape``.  How each module is pinned is listed in its
+own docstring, with the tests that do it (``tests/test_oracle_*.py``,
+``tests/test_host_*.py``); a function without an independent pin says
 "parity unpinned" in its docstring.
 """

@@ -7,8 +7,10 @@ Restates ``igm/utils/gradient/compute_divflux_slope_limiter.py:3-65``,
 Pins (``tests/test_oracle_transport.py``): exact conservation of the flux
 divergence (it sums to the boundary fluxes, which vanish for ice away from
 the edges), a uniform-velocity translation of a smooth profile against the
-exact shifted profile, godunov = first-order upwind closed form, and
-limiter identities (minmod/superbee on monotone and extremal data).
+exact shifted profile, godunov = first-order upwind closed form, limiter
+identities (minmod/superbee on monotone and extremal data), the order of
+accuracy on a smooth front (2 with superbee, 1 with godunov) and the TVD
+property of the limited schemes on square waves.
 """
 import numpy as np
 

@@ -2,7 +2,7 @@
 
 Restates ``igm/processes/iceflow/vert_disc.py``.  Pinned by the closed forms
 for Nz=10, vert_spacing=4 (levels_k = k(k+3)/108, dzeta_l = (l+2)/54,
-vert_weight_k = (6k+13)/400) checked in ``tests/test_oracle_vert_disc.py``.
+vert_weight_k = (6k+13)/400) checked in ``tests/test_oracle_misc.py``.
 """
 import numpy as np
 

@@ -1,0 +1,163 @@
+"""Write tests/golden/dome100_minimiser.npz: the float64 minimiser of the
+oracle's Blatter-Pattyn energy on the dome100 workload (100 x 100, dx 100 m,
+one dome R = 4 km, H = 300 m on a random-field bed, A = 78, C = 0.0464,
+Nz = 10, vert_spacing 4).
+
+The energy is convex, so the minimiser is unique; the reference's solver
+(legacy Adam, solve/solve.py:22-119) approaches it.  Here it is computed
+independently of that method, on the CPU, from the oracle's energy and its
+autograd gradient (oracle/energy.py, float64):
+
+1. L-BFGS from U = V = 0 (scipy) until the energy settles;
+2. cycles of L-BFGS in preconditioned variables x = x_c + L^-T y, where
+   L L^T is the block diagonal of the Hessian at the cycle's start point,
+   one 20 x 20 block per node column (U and V at the Nz levels; the
+   vertical coupling is the stiffest part of the problem).  The blocks come
+   from Hessian-vector products with probe vectors that colour the grid
+   by (j mod 3, i mod 3), which no two nodes of one cell share.
+
+The file records the field (float64), its energy, the gradient norm, and
+the relative change of the field over each cycle's last 500 iterations
+(``tail_change``): the last one bounds how far the field can still be from
+the minimiser.  Run time: about 30 minutes on 8 cores.
+
+    python tests/golden/make_dome100_minimiser.py [--cycles N]
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+from scipy.optimize import minimize
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from igm_amd import synthetic as syn  # noqa: E402
+from oracle import energy as E  # noqa: E402
+from oracle import vert_disc as VD  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dome100_minimiser.npz")
+NZ = 10
+
+
+class Problem:
+    def __init__(self):
+        g = syn.dome100()
+        self.thk = g["thk"]
+        self.ny, self.nx = g["thk"].shape
+        self.fin = E.fieldin_tensors(*(g[k] for k in ("thk", "usurf", "arrhenius", "slidingco", "dX")),
+                                     dtype=torch.float64)
+        self.dz = torch.as_tensor(VD.compute_zeta_dzeta(VD.compute_levels(NZ, 4.0))[1], dtype=torch.float64)
+        self.ice = torch.as_tensor(self.thk > 0)
+
+    def J(self, x):
+        return E.total_cost(x[0][None], x[1][None], self.fin, self.dz)[0]
+
+    def grad(self, x):
+        x = x.detach().requires_grad_(True)
+        c = self.J(x)
+        (gx,) = torch.autograd.grad(c, x)
+        return float(c.detach()), gx
+
+    def column_blocks(self, x):
+        """(ny, nx, 2Nz, 2Nz) Hessian blocks of the node columns at x."""
+        ny, nx = self.ny, self.nx
+        xr = x.detach().requires_grad_(True)
+        (gx,) = torch.autograd.grad(self.J(xr), xr, create_graph=True)
+        B = torch.zeros(ny, nx, 2 * NZ, 2 * NZ, dtype=torch.float64)
+        Jj, I = torch.meshgrid(torch.arange(ny), torch.arange(nx), indexing="ij")
+        for cj in range(3):
+            for ci in range(3):
+                sel = (Jj % 3 == cj) & (I % 3 == ci)
+                for c in range(2):
+                    for k in range(NZ):
+                        v = torch.zeros(2, NZ, ny, nx, dtype=torch.float64)
+                        v[c, k][sel] = 1.0
+                        (h,) = torch.autograd.grad(gx, xr, grad_outputs=v, retain_graph=True)
+                        B[sel, :, c * NZ + k] = h.permute(2, 3, 0, 1).reshape(ny, nx, 2 * NZ)[sel]
+        return B
+
+    def rel_change(self, a, b):
+        return float((a - b)[:, :, self.ice].norm() / b[:, :, self.ice].norm())
+
+
+def lbfgs(fun, y0, iters, log, every=500):
+    """scipy L-BFGS for ``iters`` iterations; returns (y, [iterates every ``every``])."""
+    snaps = []
+
+    def cb(yk):
+        cb.n += 1
+        if cb.n % every == 0:
+            snaps.append(yk.copy())
+            log(cb.n)
+    cb.n = 0
+    r = minimize(fun, y0, jac=True, method="L-BFGS-B", callback=cb,
+                 options=dict(maxiter=iters, maxcor=30, gtol=1e-30, ftol=1e-30, maxfun=10 ** 9))
+    return r.x, snaps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--plain-its", type=int, default=3000)
+    ap.add_argument("--cycles", type=int, default=4)
+    ap.add_argument("--cycle-its", type=int, default=3000)
+    args = ap.parse_args()
+    torch.set_num_threads(min(8, os.cpu_count() or 1))
+    P = Problem()
+    n = 2 * NZ * P.ny * P.nx
+    shape = (2, NZ, P.ny, P.nx)
+    t0 = time.time()
+
+    def plain(v):
+        c, gx = P.grad(torch.as_tensor(v).view(shape))
+        return c, gx.numpy().ravel()
+
+    x = torch.zeros(shape, dtype=torch.float64)
+    y, _ = lbfgs(plain, x.numpy().ravel(), args.plain_its,
+                 lambda k: print(f"plain L-BFGS {k} its ({time.time() - t0:.0f} s)", flush=True))
+    x = torch.as_tensor(y).view(shape).clone()
+    tail = []
+    for cyc in range(args.cycles):
+        B = P.column_blocks(x)
+        act = torch.diagonal(B, dim1=-2, dim2=-1) > 0
+        B = B * (act[..., :, None] & act[..., None, :]).double() + torch.diag_embed((~act).double())
+        B = 0.5 * (B + B.transpose(-1, -2))
+        ev = torch.linalg.eigvalsh(B)
+        B = B + torch.diag_embed((1e-12 * ev.max(-1).values)[..., None].expand(-1, -1, 2 * NZ))
+        Mt = torch.linalg.inv(torch.linalg.cholesky(B)).transpose(-1, -2)  # L^-T per column
+        mask = act.reshape(P.ny, P.nx, 2, NZ).permute(2, 3, 0, 1)
+        xc = x.clone()
+
+        def to_x(v, xc=xc, Mt=Mt, mask=mask):
+            yy = torch.as_tensor(v).view(P.ny, P.nx, 2 * NZ, 1)
+            return xc + (Mt @ yy).reshape(P.ny, P.nx, 2, NZ).permute(2, 3, 0, 1) * mask
+
+        def fun(v, Mt=Mt, mask=mask, to_x=to_x):
+            c, gx = P.grad(to_x(v))
+            gy = Mt.transpose(-1, -2) @ (gx * mask).permute(2, 3, 0, 1).reshape(P.ny, P.nx, 2 * NZ, 1)
+            return c, gy.reshape(-1).numpy()
+
+        def log(k, cyc=cyc):
+            print(f"cycle {cyc}: {k} its ({time.time() - t0:.0f} s)", flush=True)
+
+        y, snaps = lbfgs(fun, np.zeros(n), args.cycle_its, log)
+        x_new = to_x(y)
+        last = to_x(snaps[-2]) if len(snaps) >= 2 else xc
+        tail.append(P.rel_change(x_new, last))
+        c, gx = P.grad(x_new)
+        print(f"cycle {cyc}: J {c:.15f} |grad| {float(gx.norm()):.3e} change over its last 500 its "
+              f"{tail[-1]:.2e}, over the cycle {P.rel_change(x_new, x):.2e}", flush=True)
+        x = x_new
+    c, gx = P.grad(x)
+    _, g0 = P.grad(torch.zeros(shape, dtype=torch.float64))
+    np.savez_compressed(OUT, U=x[0].numpy(), V=x[1].numpy(), J=c, grad_norm=float(gx.norm()),
+                        grad_norm_at_zero=float(g0.norm()), tail_change=np.asarray(tail))
+    print(f"wrote {OUT}: J {c:.15f}, |grad| {float(gx.norm()):.3e} (at zero {float(g0.norm()):.3e}), "
+          f"tail changes {tail}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

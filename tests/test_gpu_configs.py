@@ -1,0 +1,257 @@
+"""Every workload of the benchmark suite at its own size on the GPU.
+
+* domes400 (400 x 400, dx 50 m, four domes, per-cell arrhenius in [50, 100]):
+  float32 energy and gradient against the float64 oracle on the whole
+  field; central differences of the float64 kernel at random nodes; a
+  converged float32 solve against the converged float64 one.
+* domes1000 (1000 x 1000, twenty domes): energy and gradient against the
+  oracle on the whole field; the emulator's field finite and masked.
+* patch_batch (32 patches of sides 128..512): the gradient of the padded,
+  masked batch equals the sum of the per-patch gradients; the training loss
+  falls.
+* domes1500 (1500 x 1500): a short transient run with online retraining and
+  the energy-gap check.
+
+Tolerances are written in each test.
+"""
+import numpy as np
+import pytest
+import torch
+
+from igm_amd import emulator as EM
+from igm_amd import synthetic as syn
+from igm_amd.energy import EnergyOperator
+from igm_amd.solver import LegacyAdamState, adam_schedule, rel_err, solve
+from oracle import energy as E
+from tests.helpers import dzeta_for, rel
+
+pytestmark = pytest.mark.gpu
+
+PHYS = E.DEFAULT_PHYSICS
+KEYS = ("thk", "usurf", "arrhenius", "slidingco", "dX")
+NZ = 10
+
+
+def device_fields(g, dev, dtype=torch.float32):
+    return {k: torch.as_tensor(g[k], device=dev).to(dtype) for k in KEYS}
+
+
+def oracle_energy_grad(g, U, V):
+    fin = E.fieldin_tensors(*(g[k] for k in KEYS), dtype=torch.float64)
+    dz = torch.as_tensor(dzeta_for(NZ), dtype=torch.float64)
+    c, means, gU, gV = E.energy_and_grad(U.cpu().double(), V.cpu().double(), fin, dz)
+    return float(c), [float(m) for m in means], gU, gV
+
+
+def realistic_velocity(op32, n=300):
+    """A velocity field with the structure of a solution: 300 solver
+    iterations from zero (lr 300 -> 30)."""
+    z = torch.zeros((NZ, op32.Ny, op32.Nx), dtype=torch.float32, device=op32.device)
+    r = solve(op32, z, z.clone(), n, LegacyAdamState(lr=300.0, beta1=0.95, beta2=0.95), early_stopping=False,
+              lr_schedule=adam_schedule(300.0, n // 5, n - n // 5, 30.0))
+    return r.U.contiguous(), r.V.contiguous()
+
+
+def check_energy_grad(g, op32, U, V):
+    c_ref, m_ref, gU_ref, gV_ref = oracle_energy_grad(g, U, V)
+    e, gU, gV = op32.energy_grad(U, V)
+    e = e.cpu().numpy()[0]
+    scale = sum(abs(m) for m in m_ref)
+    print(f"  COST gpu {e[3]:.9f} oracle {c_ref:.9f}", flush=True)
+    for k in range(3):
+        assert abs(e[k] - m_ref[k]) <= 2e-6 * scale, (k, e[k], m_ref[k])
+    assert abs(e[3] - c_ref) <= 2e-6 * scale
+    er = rel(torch.stack([gU.cpu(), gV.cpu()]), torch.stack([gU_ref, gV_ref]))
+    print(f"  gradient rel L2 {er:.2e}", flush=True)
+    assert er <= 2e-5
+
+
+@pytest.fixture(scope="module")
+def domes400(dev):
+    g = syn.domes400()
+    assert g["thk"].shape == (400, 400) and g["dx"] == 50.0
+    assert g["arrhenius"].min() >= 50.0 and g["arrhenius"].max() <= 100.0 and g["arrhenius"].std() > 10
+    op32 = EnergyOperator(dzeta_for(NZ), PHYS, fields=device_fields(g, dev))
+    return g, op32
+
+
+def test_domes400_energy_gradient_vs_oracle(dev, domes400):
+    print("case domes400 energy+gradient vs oracle", flush=True)
+    g, op32 = domes400
+    U, V = realistic_velocity(op32)
+    assert float(U.abs().max()) > 10.0
+    check_energy_grad(g, op32, U, V)
+
+
+def test_domes400_central_differences_f64(dev, domes400):
+    """Central differences of the float64 kernel's energy at random ice
+    nodes (interior and margins) and random levels against its gradient."""
+    print("case domes400 central differences", flush=True)
+    g, op32 = domes400
+    U, V = realistic_velocity(op32)
+    op = EnergyOperator(dzeta_for(NZ), PHYS, fields=device_fields(g, dev, torch.float64), dtype=torch.float64)
+    Ud, Vd = U.double().contiguous(), V.double().contiguous()
+    _, gU, gV = op.energy_grad(Ud, Vd)
+    ice = np.argwhere(g["thk"] > 0)
+    rng = np.random.default_rng(2)
+    for n in range(16):
+        j, i = ice[rng.integers(len(ice))]
+        k, which = int(rng.integers(NZ)), n % 2
+        T = Ud if which == 0 else Vd
+        h = 1e-3 * max(1.0, abs(float(T[k, j, i])))
+        o = float(T[k, j, i])
+        T[k, j, i] = o + h
+        ep = float(op.energy(Ud, Vd)[0, 3])
+        T[k, j, i] = o - h
+        em = float(op.energy(Ud, Vd)[0, 3])
+        T[k, j, i] = o
+        fd = (ep - em) / (2 * h)
+        an = float((gU if which == 0 else gV)[k, j, i])
+        scale = float(torch.linalg.vector_norm(gU)) / np.sqrt(gU.numel())
+        assert abs(fd - an) <= 1e-5 * max(abs(an), 1e-3 * scale), (which, k, j, i, fd, an)
+
+
+def test_domes400_converged_float32_equals_converged_float64(dev, domes400):
+    """The energy is convex, so its minimiser is unique: float32 and float64
+    solves driven to convergence (decaying steps) must meet, within 1e-4
+    relative L2 on the ice, and the oracle's float64 gradient at the float64
+    field must be tiny against its gradient at zero."""
+    print("case domes400 converged solve", flush=True)
+    g, op32 = domes400
+    op64 = EnergyOperator(dzeta_for(NZ), PHYS, fields=device_fields(g, dev, torch.float64), dtype=torch.float64)
+    z = torch.zeros((NZ, 400, 400), dtype=torch.float32, device=dev)
+    n = 12000
+    r32 = solve(op32, z, z.clone(), n, LegacyAdamState(lr=1.0), early_stopping=False,
+                lr_schedule=adam_schedule(10.0, 3000, n - 3000, 1e-4))
+    r64 = solve(op64, r32.U.double(), r32.V.double(), 4000, LegacyAdamState(lr=1.0), early_stopping=False,
+                lr_schedule=adam_schedule(1e-2, 0, 4000, 1e-7))
+    mask = torch.as_tensor(g["thk"] > 0, device=dev)
+    d = rel_err(r32.U, r32.V, r64.U, r64.V, mask)
+    print(f"  float32 vs float64 minimiser: rel L2 {d:.2e}", flush=True)
+    assert d <= 1e-4
+    _, _, g64U, g64V = oracle_energy_grad(g, r64.U, r64.V)
+    zc = torch.zeros((NZ, 400, 400), dtype=torch.float64)
+    _, _, g0U, g0V = oracle_energy_grad(g, zc, zc)
+    ratio = float(torch.linalg.vector_norm(torch.stack([g64U, g64V])) / torch.linalg.vector_norm(
+        torch.stack([g0U, g0V])))
+    print(f"  oracle |grad| at the float64 field / at zero: {ratio:.2e}", flush=True)
+    assert ratio <= 1e-3
+
+
+def test_domes1000_energy_gradient_and_emulator(dev):
+    print("case domes1000 energy+gradient vs oracle, emulator inference", flush=True)
+    g = syn.domes1000()
+    assert g["thk"].shape == (1000, 1000)
+    f32 = device_fields(g, dev)
+    op32 = EnergyOperator(dzeta_for(NZ), PHYS, fields=f32)
+    weights, _ = EM.load_weights(EM.os.path.join(EM.EMULATOR_DIR, "igm_amd_cnn_16_32_nz10.safetensors"))
+    cnn = EM.CNN(NZ, 16, device=dev, weights=weights)
+    Y = cnn.forward(EM.planar_fields([f32[k] for k in KEYS]))
+    assert Y.shape == (1, 1000, 1000, 2 * NZ) and bool(torch.isfinite(Y).all())
+    from igm_amd.fields import velocity_outputs
+    from igm_amd.vert_disc import define_vertical_weight
+    out = velocity_outputs(Y, None, f32["thk"], define_vertical_weight(NZ, 4.0), NZ)
+    ice = f32["thk"] > 0
+    assert float(out["U"][:, ~ice].abs().max()) == 0.0 and float(out["ubar"][ice].abs().max()) > 10.0
+    check_energy_grad(g, op32, out["U"].contiguous(), out["V"].contiguous())
+
+
+def patch_grads(cnn, parts, dz, dev):
+    """Sum over the patches of their own gradients (one patch per launch)."""
+    total = torch.zeros_like(cnn.g)
+    for p in parts:
+        X = p[None].contiguous()
+        op = EnergyOperator(dz, PHYS, X=X)
+        fields = EM.nhwc_fields(X)
+        Y, acts = cnn.forward(fields, keep=True)
+        _, gY = op.energy_grad(Y)
+        cnn.backward(gY, acts, fields)
+        total += cnn.g
+    return total
+
+
+def test_patch_batch_gradient_is_the_sum_of_patch_gradients(dev):
+    """One zero-padded batch of the 32 patches (sides 128..512), masked by
+    extents, in one launch per layer: its weight gradient equals the sum of
+    the 32 single-patch gradients (float32 sums in another order: 1e-5)."""
+    print("case patch_batch(0) batched vs per-patch gradient", flush=True)
+    from igm_amd.pretraining import PatchBatchBatcher
+    dz = dzeta_for(NZ)
+    cnn = EM.CNN(NZ, 16, device=dev, seed=3)
+    (X, ext), = PatchBatchBatcher(dev, start=0).next()
+    sides = ext.cpu().numpy()
+    assert X.shape[0] == 32 and sides.min() >= 128 and sides.max() <= 512 and len(set(sides[:, 0])) > 10
+    op = EnergyOperator(dz, PHYS, X=X, extents=ext)
+    fields = EM.nhwc_fields(X)
+    Y, acts = cnn.forward(fields, extents=ext, keep=True)
+    _, gY = op.energy_grad(Y)
+    cnn.backward(gY, acts, fields, extents=ext)
+    g_batch = cnn.g.clone()
+    parts = [X[b, :int(sides[b, 0]), :int(sides[b, 1])].contiguous() for b in range(32)]
+    g_sum = patch_grads(cnn, parts, dz, dev)
+    d = rel(g_batch.cpu(), g_sum.cpu())
+    print(f"  rel L2 batch vs sum of patches {d:.2e}", flush=True)
+    assert d <= 1e-5
+
+
+def test_patch_batch_training_lowers_the_loss(dev):
+    """Twenty training steps on fresh batches patch_batch(0..19) at lr 1e-4
+    from a random initialisation lower the mean COST of a held-out batch."""
+    print("case patch_batch training", flush=True)
+    from igm_amd import pretraining as PT
+    dz = dzeta_for(NZ)
+    cnn = EM.CNN(NZ, 16, device=dev, seed=11)
+    (Xh, exth), = PT.PatchBatchBatcher(dev, start=1000).next()
+    oph = EnergyOperator(dz, PHYS, X=Xh, extents=exth)
+
+    def held_out():
+        Y = cnn.forward(EM.nhwc_fields(Xh), extents=exth)
+        return float(oph.energy(Y)[:, 3].mean())
+
+    before = held_out()
+    PT.train(cnn, PT.PatchBatchBatcher(dev, start=0, buckets=4), dz, PHYS, steps=20, lr=1e-4, log_every=10)
+    after = held_out()
+    print(f"  held-out mean COST {before:.5f} -> {after:.5f}", flush=True)
+    assert after < before - 0.05 * abs(before)
+
+
+def test_domes1500_transient_steps(dev):
+    """The transient workload at its size: twelve steps of smb_simple ->
+    iceflow (emulated, retraining at steps 0 and 10, the energy check every
+    5th update) -> time -> thk.  Thickness stays >= 0 and finite, the steps
+    obey CFL 0.3 on the previous velocities, and the flux divergence
+    conserves mass (the ice is away from the edges)."""
+    print("case domes1500 transient", flush=True)
+    from igm_amd import transient as TR
+    g = syn.domes1500()
+    assert g["thk"].shape == (1500, 1500)
+    cfg = TR.transient_cfg(method="emulated", years=1e6, retrain_freq=10, check_its=20, check_freq=5)
+    st = TR.make_state(g)
+    from igm_amd import iceflow
+    from igm_amd.processes import smb_simple, thk as thkmod, time as timemod
+    thkmod.initialize(cfg, st)
+    timemod.initialize(cfg, st)
+    smb_simple.initialize(cfg, st)
+    iceflow.initialize(cfg, st)
+    st.it = 0
+    vols = []
+    for step in range(12):
+        smb_simple.update(cfg, st)
+        iceflow.update(cfg, st)
+        vmax = max(float(st.ubar.abs().max()), float(st.vbar.abs().max()))
+        timemod.update(cfg, st)
+        assert float(st.dt) <= min(0.3 * 50.0 / vmax, 1.0) * (1 + 1e-5) + 1e-7
+        thk_before = st.thk.clone()
+        thkmod.update(cfg, st)
+        div = st.divflux.double()
+        assert abs(float(div.sum())) <= 1e-5 * float(div.abs().sum())
+        assert float(st.thk.min()) >= 0.0 and bool(torch.isfinite(st.thk).all())
+        # where nothing was clipped at zero, thk changed by dt (smb - div q)
+        pred = thk_before.double() + float(st.dt) * (st.smb.double() - div)
+        ok = pred > 1.0
+        assert float((st.thk.double() - pred)[ok].abs().max()) <= 1e-3
+        vols.append(float(st.thk.double().sum()) * 50.0 ** 2 / 1e9)
+        st.it += 1
+    assert len(st.COST_EMULATOR) >= 1 and np.isfinite(st.COST_EMULATOR[-1])
+    assert hasattr(st, "COST_CHECK") and np.isfinite(st.COST_CHECK)
+    print(f"  volumes {vols[0]:.3f} .. {vols[-1]:.3f} km3, fallbacks {st.fallback_count}", flush=True)

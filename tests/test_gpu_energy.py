@@ -211,3 +211,47 @@ def test_deterministic_bitwise(dev):
     r2 = op.energy_grad(Ud, Vd)
     for a, b in zip(r1, r2):
         assert torch.equal(a, b)
+
+
+def floating_glacier(ny, nx, seed):
+    """small_glacier with a sliding field that is zero on one block, negative
+    on another and mixed in sign along their edges, so that the cell average
+    of slidingco is <= 0 (floating-ice damping, cost_shear.py:85-90) on whole
+    blocks and on cells with some positive corners."""
+    g = small_glacier(ny, nx, seed=seed)
+    rng = np.random.default_rng(seed)
+    C = np.full((ny, nx), 0.0464, np.float32)
+    C[ny // 4:ny // 2, nx // 5:nx // 2] = 0.0
+    C[ny // 2:, nx // 2:(3 * nx) // 4] = -0.03
+    # scattered single negative corners (some outweigh their three positive neighbours)
+    for _ in range(max(1, ny * nx // 40)):
+        C[rng.integers(ny), rng.integers(nx)] = rng.choice([-0.2, -0.01, 0.0])
+    g["slidingco"] = C
+    return g
+
+
+@pytest.mark.parametrize("ny,nx", [(17, 61), (40, 64), (50, 124)])
+@pytest.mark.parametrize("dtype,ctol,gtol", [(torch.float32, 2e-6, 2e-5), (torch.float64, 1e-11, 1e-11)])
+def test_floating_damping_vs_oracle(dev, ny, nx, dtype, ctol, gtol):
+    nz = 10
+    print(f"case floating damping {dtype} ny={ny} nx={nx}", flush=True)
+    g = floating_glacier(ny, nx, seed=ny + nx)
+    C = g["slidingco"]
+    cs = (C[:-1, :-1] + C[:-1, 1:] + C[1:, :-1] + C[1:, 1:]) / 4
+    assert (cs <= 0).sum() >= 20 and (cs > 0).sum() >= 20
+    U, V = smooth_velocity(nz, ny, nx, seed=ny)
+    c_ref, m_ref, gU_ref, gV_ref = oracle_eval(g, U, V, nz)
+    op = gpu_op(g, nz, dev, dtype)
+    e, gU, gV = op.energy_grad(to_dev(U, dev, dtype), to_dev(V, dev, dtype))
+    e = e.cpu().numpy()[0]
+    scale = sum(abs(m) for m in m_ref)
+    for k in range(3):
+        assert abs(e[k] - m_ref[k]) <= ctol * scale, (k, e[k], m_ref[k])
+    assert abs(e[3] - c_ref) <= ctol * scale
+    assert rel(torch.stack([gU.cpu(), gV.cpu()]), torch.stack([gU_ref, gV_ref])) <= gtol
+    # the damped cells change the energy: the same field with slidingco
+    # clipped to > 0 has a different shear energy
+    g2 = dict(g)
+    g2["slidingco"] = np.maximum(C, np.float32(1e-3))
+    _, m2, _, _ = oracle_eval(g2, U, V, nz)
+    assert abs(m2[0] - m_ref[0]) > 100 * ctol * scale

@@ -144,3 +144,69 @@ def test_refine_polishes_with_its_schedule_and_adam_betas(dev):
                       lr_schedule=slv.adam_schedule(3.0, 3, 27, 0.3))
     assert not torch.equal(r_def.U, r.U)
     assert state.COST_REFINED < float(op.energy(U0, V0)[0, 3])
+
+
+def test_fallback_replaces_a_failed_emulated_field(dev):
+    """Energy-gap fallback: an emulator with zeroed weights predicts U = V = 0,
+    whose energy lies far above what a few solver iterations reach, so the
+    check falls back: fallback_count goes up by one per update and the
+    module's U, V are the solver's field (the check's iterations, then the
+    fallback solve with EarlyStopping, on the module's own optimizer),
+    masked to the ice.  A second update with the same zero field falls back
+    again."""
+    from igm_amd import solver as slv
+    cfg = config.default_cfg()
+    ice = cfg.processes.iceflow
+    ice.method = "emulated"
+    ice.emulator.retrain_freq = 0
+    fb = ice.emulator.fallback
+    fb.check_its, fb.max_its = 10, 300
+    g = syn.dome100()
+    state = State()
+    for k in ("thk", "usurf", "dX"):
+        setattr(state, k, g[k])
+    state.it = -1
+    iceflow.initialize(cfg, state)
+    model = state.iceflow_model
+    model.set_weights([(np.zeros_like(k), np.zeros_like(b)) for k, b in model.get_weights()])
+    for call in range(2):
+        state.it = call
+        t0 = state.optimizer.iterations
+        iceflow.update(cfg, state)
+        assert state.fallback_count == call + 1
+        assert float(state.COST_EMULATED) > state.COST_CHECK + 0.01 * abs(state.COST_CHECK)
+        # the same two solves on a copy of the optimizer at the same step
+        op = iceflow.energy_operator(cfg, state)
+        z = torch.zeros_like(state.U)
+        opt = slv.LegacyAdamState(lr=float(ice.solver.step_size))
+        opt.iterations = t0
+        r1 = slv.solve(op, z, z.clone(), 10, opt, early_stopping=False)
+        r2 = slv.solve(op, r1.U, r1.V, 300, opt, early_stopping=True)
+        assert opt.iterations == state.optimizer.iterations
+        ice_cells = state.thk > 0
+        for got, want in ((state.U, r2.U), (state.V, r2.V)):
+            assert torch.equal(got, torch.where(ice_cells, want, torch.zeros_like(want)))
+        assert float(state.ubar.abs().max()) > 1.0
+
+
+def test_fallback_keeps_a_good_emulated_field(dev):
+    """The same check on a field that is already the solver's minimiser
+    (here: the polished emulated field, then the check) does not fall back."""
+    cfg = config.default_cfg()
+    ice = cfg.processes.iceflow
+    ice.method = "emulated"
+    ice.emulator.retrain_freq = 0
+    rf = ice.emulator.refine
+    rf.its, rf.lr, rf.flat_frac, rf.lr_end, rf.beta1, rf.beta2 = 600, 30.0, 0.3, 0.01, 0.95, 0.99
+    g = syn.dome100()
+    state = State()
+    for k in ("thk", "usurf", "dX"):
+        setattr(state, k, g[k])
+    state.it = -1
+    iceflow.initialize(cfg, state)
+    state.it = 0
+    state.fallback_count = 0
+    iceflow.update(cfg, state)
+    assert state.fallback_count == 0
+    gap = (float(state.COST_EMULATED) - state.COST_CHECK) / abs(state.COST_CHECK)
+    assert gap <= 0.01

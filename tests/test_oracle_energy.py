@@ -172,3 +172,70 @@ def test_float32_matches_float64():
     c64, _ = E.total_cost(U[None], V[None], fin, dz)
     c32, _ = E.total_cost(U.float()[None], V.float()[None], fin32, dz.float())
     assert float(c32) == pytest.approx(float(c64), rel=1e-5)
+
+
+def _shear_slab(C, H=300.0, ax=0.0, ay=0.0, gamma=0.02, dx=100.0):
+    """A slab of uniform thickness H on a plane bed of slopes (ax, ay), with
+    the bed-parallel simple shear u = gamma (z - b): U_k = gamma H levels_k,
+    V = 0.  Returns (fieldin, U, V)."""
+    ny, nx = C.shape
+    X, Y = np.meshgrid(np.arange(nx) * dx, np.arange(ny) * dx)
+    bed = 2500.0 + ax * X + ay * Y
+    thk = np.full((ny, nx), H)
+    fin = E.fieldin_tensors(thk, bed + thk, np.full((ny, nx), 78.0), C, np.full((ny, nx), dx), torch.float64)
+    lev = torch.as_tensor(VD.compute_levels(NZ, 4.0), dtype=torch.float64)
+    U = (gamma * H * lev)[:, None, None].expand(NZ, ny, nx).clone()
+    return fin, U, torch.zeros_like(U)
+
+
+@pytest.mark.parametrize("ax,ay", [(0.0, 0.0), (0.05, 0.0), (0.03, -0.04)])
+def test_floating_damping_closed_form(ax, ay):
+    """cost_shear.py:85-90: where the cell average of slidingco is <= 0, the
+    vertical derivatives are scaled by 0.01 before the change of coordinate
+    z -> z - b.  For the bed-parallel simple shear u = g (z - b) the
+    Blatter-Pattyn strain-rate tensor has e_xz = g/2, e_xx = -e_zz = -g bx,
+    e_xy = -g by / 2, so eps_e^2 = g^2 (bx^2 + by^2/4 + 1/4), and each cell's
+    shear energy is H B (3/4) (eps_e^2 + regu^2)^(-1/3) eps_e^2 with g the
+    shear rate, times 0.01 in the damped cells.  The sliding field has a
+    block of zeros, a block of negative values and mixed-sign edges, so cells
+    with some positive corners but a non-positive average are damped, and
+    cells with a zero corner but a positive average are not."""
+    ny, nx = 9, 11
+    C = np.full((ny, nx), 0.0464)
+    C[2:5, 3:7] = 0.0          # zero block: interior cells have average 0 -> damped
+    C[6:9, 0:3] = -0.05        # negative block
+    C[6, 3] = 0.01             # one positive corner next to the negative block
+    C[0, 10] = -0.2            # one negative corner outweighs three positive ones: damped
+    gamma, H = 0.02, 300.0
+    fin, U, V = _shear_slab(C, H=H, ax=ax, ay=ay, gamma=gamma)
+    m = E.cost_shear(U[None], V[None], *fin, dz64(), E.DEFAULT_PHYSICS)[0].numpy()
+    B = 2.0 * 78.0 ** (-1.0 / 3.0)
+    ndamp = nund = 0
+    for j in range(ny - 1):
+        for i in range(nx - 1):
+            cs = (C[j, i] + C[j, i + 1] + C[j + 1, i] + C[j + 1, i + 1]) / 4
+            g = gamma * (1.0 if cs > 0 else 0.01)
+            eps2 = g * g * (ax * ax + ay * ay / 4 + 0.25)
+            expect = H * B * 0.75 * (eps2 + 1e-10) ** (-1.0 / 3.0) * eps2
+            assert m[j, i] == pytest.approx(expect, rel=1e-10), (j, i, cs)
+            if cs > 0:
+                nund += 1
+            else:
+                ndamp += 1
+    assert ndamp >= 10 and nund >= 10
+
+
+def test_floating_damping_scales_shear_like_slower_ice():
+    """With no slope terms the damped cell's shear energy equals the undamped
+    energy of the field scaled by 0.01, and with regu_glen small against the
+    strain rate the ratio is the power law's 0.01^(4/3)."""
+    ny, nx = 4, 4
+    fin_g, U, V = _shear_slab(np.full((ny, nx), 0.0464), gamma=0.1)
+    fin_f, _, _ = _shear_slab(np.zeros((ny, nx)), gamma=0.1)
+    dz = dz64()
+    grounded = E.cost_shear(U[None], V[None], *fin_g, dz, E.DEFAULT_PHYSICS)
+    floating = E.cost_shear(U[None], V[None], *fin_f, dz, E.DEFAULT_PHYSICS)
+    slow = E.cost_shear(0.01 * U[None], V[None], *fin_g, dz, E.DEFAULT_PHYSICS)
+    assert torch.allclose(floating, slow, rtol=1e-13, atol=0)
+    # (regu_glen^2 = 1e-10 against the damped eps_e^2 = 2.5e-7 moves the ratio by ~1.3e-4)
+    assert float(floating.mean() / grounded.mean()) == pytest.approx(0.01 ** (4.0 / 3.0), rel=5e-4)
