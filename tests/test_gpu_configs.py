@@ -92,11 +92,14 @@ def test_domes400_central_differences_f64(dev, domes400):
     op = EnergyOperator(dzeta_for(NZ), PHYS, fields=device_fields(g, dev, torch.float64), dtype=torch.float64)
     Ud, Vd = U.double().contiguous(), V.double().contiguous()
     _, gU, gV = op.energy_grad(Ud, Vd)
-    ice = np.argwhere(g["thk"] > 0)
     rng = np.random.default_rng(2)
+    rms = float(torch.linalg.vector_norm(torch.stack([gU, gV]))) / np.sqrt(2 * gU.numel())
     for n in range(16):
-        j, i = ice[rng.integers(len(ice))]
-        k, which = int(rng.integers(NZ)), n % 2
+        which = n % 2
+        G = (gU if which == 0 else gV).cpu().numpy()
+        # ice nodes whose gradient entry is not negligible against the field's
+        cand = np.argwhere((np.abs(G) > 0.1 * rms) & (g["thk"] > 0)[None])
+        k, j, i = cand[rng.integers(len(cand))]
         T = Ud if which == 0 else Vd
         h = 1e-3 * max(1.0, abs(float(T[k, j, i])))
         o = float(T[k, j, i])
@@ -107,8 +110,9 @@ def test_domes400_central_differences_f64(dev, domes400):
         T[k, j, i] = o
         fd = (ep - em) / (2 * h)
         an = float((gU if which == 0 else gV)[k, j, i])
-        scale = float(torch.linalg.vector_norm(gU)) / np.sqrt(gU.numel())
-        assert abs(fd - an) <= 1e-5 * max(abs(an), 1e-3 * scale), (which, k, j, i, fd, an)
+        # truncation O(h^2), and the rounding of a float64 sum over 160,000
+        # cells, below 1e-6 of the gradient's rms
+        assert abs(fd - an) <= 1e-5 * abs(an) + 1e-6 * rms, (which, k, j, i, fd, an)
 
 
 def test_domes400_converged_float32_equals_converged_float64(dev, domes400):
