@@ -55,6 +55,8 @@ _SIGS = {
     "igm_energy_pass2_f32": (C.c_int, [I32, I32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
                                        VP, VP, I32, VP, VP, VP, VP, I32, VP]),
     "igm_energy_finalize_pair": (C.c_int, [VP, I32, I32, VP, VP, VP, I32, VP]),
+    "igm_energy_eval_f32_noalloc": (C.c_int, [I32, I32, I32, I32, I32, VP, VP, I64, I64, I64, I64, VP, VP, VP,
+                                              VP, VP, VP, VP, VP]),
     "igm_energy_solve_pass_f32": (C.c_int, [I32, I32, I32, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
                                             I32, VP, VP, VP, VP, VP, VP, I32, VP]),
     "igm_velocity_outputs_f32": (C.c_int, [I32, I32, I32, VP, VP, I64, I64, I64, VP, VP, VP, VP, VP, VP, VP,

@@ -580,11 +580,80 @@ int velout_impl2(int nz, int Ny, int Nx, const T* U, const T* V, int64_t ks, int
   IGM_CHECK_LAUNCH();
   return 0;
 }
+// ------------------------------------------- allocation-free evaluation --
+//
+// The per-block partial sums of igm_energy_eval_f32_noalloc live in this
+// static device array instead of a caller workspace: 65,536 doubles
+// (512 KiB), held by every GPU the library runs on until the process ends.
+// Host code never passes the array's name to a kernel: it looks up the
+// array's device address once per device (hipGetSymbolAddress, with that
+// device current) and passes that.  The pass overwrites every partial it
+// reads back, so nothing has to start at zero; calls that overlap on one
+// GPU race for the array.
+constexpr int64_t kScratchDoubles = 65536;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device (nullptr + error set on failure).
+double* scratch_on_current_device() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("igm_energy_eval_f32_noalloc: no current HIP device (or more than 64)");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error(std::string("igm_energy_eval_f32_noalloc: hipGetSymbolAddress failed: ") +
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace igm
 
 extern "C" {
 
 const char* igm_last_error(void) { return igm::last_error(); }
+
+int igm_energy_eval_f32_noalloc(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
+                                const float* U, const float* V, int64_t bstride, int64_t kstride,
+                                int64_t rstride, int64_t cstride, const float* coef,
+                                const float* hscale, const double* dzeta, const IgmPhysics* phys,
+                                float* gU, float* gV, double* out, void* stream) {
+  IGM_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (energy) or 1 (energy and gradient)");
+  IGM_CHECK_ARG(out != nullptr, "null pointer");
+  IGM_CHECK_ARG(nbatch >= 1 && nbatch <= 65535 && Ny >= 2 && Nx >= 2, "bad sizes");
+  const igm::PassGeom g = igm::pass_geom(nbatch, Ny, Nx, igm::kStripOut);
+  const int64_t need = 3 * (int64_t)nbatch * g.blocks;
+  if (need > igm::kScratchDoubles) {
+    igm::set_error("igm_energy_eval_f32_noalloc: this batch needs " + std::to_string(need) +
+                   " partial sums, more than the " + std::to_string(igm::kScratchDoubles) +
+                   " of the static scratch (use igm_energy_pass_f32 with a workspace)");
+    return -1;
+  }
+  double* part = igm::scratch_on_current_device();
+  if (part == nullptr) return -1;
+  const int rc = igm::pass_impl<float>(mode, nz, nbatch, Ny, Nx, U, V, bstride, kstride, rstride,
+                                       cstride, coef, hscale, dzeta, phys, gU, gV, nullptr, nullptr,
+                                       nullptr, nullptr, part, nullptr, nullptr, nullptr, nullptr,
+                                       nullptr, nullptr, 0, stream);
+  if (rc != 0) return rc;
+  hipLaunchKernelGGL(igm::energy_finalize_kernel, dim3(nbatch), dim3(igm::kFinBlock), 0,
+                     (hipStream_t)stream, part, g.blocks, 1, out, nullptr, nullptr, 0);
+  IGM_CHECK_LAUNCH();
+  return 0;
+}
 
 int64_t igm_energy_workspace_doubles(int32_t nbatch, int32_t Ny, int32_t Nx) {
   if (nbatch < 1 || Ny < 2 || Nx < 2) return 0;

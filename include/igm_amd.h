@@ -192,6 +192,23 @@ int igm_energy_solve_pass_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U
                               IgmEarlyStop* es_out, double* out, double* cost_hist,
                               int32_t finalize_only, void* stream);
 
+/* Energy (mode 0) or energy and gradient (mode 1) of (U, V), reduced into
+ * out[nbatch][4] = {mean shear, mean sliding, mean gravity, COST}: the same
+ * kernels and result, bit for bit, as igm_energy_pass_f32 (same mode, no
+ * Adam, no early stopping) followed by igm_energy_finalize, but it
+ * allocates nothing and takes no workspace.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Its per-block partial sums live in a static 512 KiB device array per GPU
+ * (65,536 doubles); a batch that needs more (3 * blocks per entry, about
+ * 800 doubles for one 1000 x 1000 field) is rejected with an error.  Calls
+ * that overlap on one GPU share that array and race. */
+int igm_energy_eval_f32_noalloc(int32_t mode, int32_t nz, int32_t nbatch, int32_t Ny, int32_t Nx,
+                                const float* U, const float* V, int64_t bstride, int64_t kstride,
+                                int64_t rstride, int64_t cstride, const float* coef,
+                                const float* hscale, const double* dzeta, const IgmPhysics* phys,
+                                float* gU, float* gV, double* out, void* stream);
+
 /* igm_energy_finalize for the partials of an igm_energy_pass2_f32 call:
  * COST and EarlyStopping (solve.py:57-59, 112) of iterations iter and
  * iter+1, in order; out[4] holds the energies of the second. */
