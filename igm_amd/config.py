@@ -16,6 +16,9 @@ from types import SimpleNamespace
 ICEFLOW = {
     "method": "emulated",
     "force_max_velbar": 0.0,
+    # build-specific: "torch" (device tensors) or "numpy" (host arrays for
+    # TensorFlow consumers of the outputs)
+    "output_format": "torch",
     "physics": {
         "energy_components": ["shear", "sliding_weertman", "gravity"],
         "sliding_law": "",
@@ -89,10 +92,11 @@ ICEFLOW = {
         },
         # build-specific energy-gap fallback to the plain solver: every
         # check_freq update() calls, check_its solver iterations (the
-        # solver's own step_size) from the emulated field give J_check, the
-        # lowest energy they see; the solve continues up to max_its
-        # iterations (with EarlyStopping) and replaces the emulated field
-        # when J_emulated > J_check + rel_energy_gap * |J_check|
+        # solver's own step_size) from the emulated field (before the
+        # thk > 0 output mask) give J_check, the lowest energy they see; the
+        # solve continues up to max_its iterations (with EarlyStopping) and
+        # replaces the emulated field when
+        # J_emulated > J_check + rel_energy_gap * |J_check|
         "fallback": {
             "enabled": True,
             "rel_energy_gap": 0.01,

@@ -346,6 +346,9 @@ def update_iceflow_emulated(cfg, state):
     out = velocity_outputs(Y, None, state.thk, state.vert_weight, nz)
     for k, v in out.items():
         setattr(state, k, v)
+    # the unmasked network output (what the energy-gap check evaluates)
+    state.iceflow_Y = Y
+    state.iceflow_unmasked = None
     if cfg.processes.iceflow.force_max_velbar > 0:
         raise NotImplementedError("force_max_velbar is not supported")
 

@@ -106,6 +106,28 @@ def _set_outputs(cfg, state, out):
         setattr(state, k, v)
 
 
+PUBLISHED = ("U", "V", "uvelbase", "vvelbase", "uvelsurf", "vvelsurf", "ubar", "vbar")
+
+
+def publish_outputs(cfg, state):
+    """Hand the output fields to the rest of the run in the form
+    ``cfg.processes.iceflow.output_format`` names: "torch" (default: device
+    tensors, for GPU consumers such as this package's thk, time and
+    vert_flow processes) or "numpy" (host float32 arrays: IGM's own
+    TensorFlow processes take them as they are, e.g. tf.abs(state.ubar) in
+    time.py:36-39 and the flux divergence in thk.py:36-38).  Inputs given
+    back as numpy at the next update are moved to the GPU again."""
+    fmt = cfgmod.get(cfg.processes.iceflow, "output_format", "torch")
+    if fmt == "torch":
+        return
+    if fmt != "numpy":
+        raise ValueError(f"output_format must be 'torch' or 'numpy', not {fmt!r}")
+    for k in PUBLISHED:
+        v = getattr(state, k, None)
+        if isinstance(v, torch.Tensor):
+            setattr(state, k, v.detach().to("cpu", torch.float32).numpy())
+
+
 def update_iceflow_solved(cfg, state):
     """solve.py:169-188."""
     ice = cfg.processes.iceflow
@@ -154,6 +176,7 @@ def initialize(cfg, state):
     if not ice.method == "solved":
         emu.update_iceflow_emulator(cfg, state, 0)
         emu.update_iceflow_emulated(cfg, state)
+    publish_outputs(cfg, state)
     state.was_initialize_iceflow_already_called = True
 
 
@@ -175,6 +198,7 @@ def update(cfg, state):
             update_iceflow_diagnostic(cfg, state)
     elif ice.method == "solved":
         update_iceflow_solved(cfg, state)
+    publish_outputs(cfg, state)
 
 
 def finalize(cfg, state):
@@ -207,6 +231,7 @@ def refine_emulated(cfg, state):
                     lr_schedule=slv.adam_schedule(lr, n_flat, n - n_flat, float(rf["lr_end"])))
     out = velocity_outputs(res.U, res.V, state.thk, state.vert_weight, int(ice.numerics.Nz))
     _set_outputs(cfg, state, out)
+    state.iceflow_unmasked = (res.U, res.V)
     state.COST_REFINED = res.costs_device[-1]  # float64 0-dim device tensor (no host wait)
 
 
@@ -215,13 +240,24 @@ def refine_emulated(cfg, state):
 def fallback_check(cfg, state):
     """Energy-gap fallback of the emulated field to the plain solver.
 
-    Every ``check_freq`` calls: J_emu = J(emulated U, V) (float64 reduction).
-    J_check is the lowest COST seen by ``check_its`` solver iterations started
-    from the emulated field.  If J_emu > J_check + rel_energy_gap * |J_check|,
-    the solve continues (up to ``max_its`` iterations with early stopping) and
-    its field replaces the emulated one.  With check_its = 0 the check only
-    records J_emu.  Records state.COST_EMULATED (a float64 0-dim device
-    tensor), state.COST_CHECK (float) and state.fallback_count.
+    Every ``check_freq`` calls: J_emu = J(U, V) of the emulated field before
+    the thk > 0 mask of the outputs (emulate.py:143-144) -- the network's own
+    output, or the polished field -- float64 reduction: the energy the
+    network is trained on and the solver minimises.  (The mask zeroes the
+    ice-free nodes of the margin cells and adds an artificial margin shear
+    that a few solver steps remove, so masked energies are not compared.)
+    Nodes with no ice within one node (all their cells ice-free) are set to
+    zero first: they enter only the sliding term of ice-free cells, which is
+    smallest at zero, so this is their optimum and no ice cell sees them,
+    while left as they are (the network's output there is unconstrained,
+    Adam leaves steps of its size there) they would dominate the gap.
+    J_check is the lowest COST seen by ``check_its`` solver iterations
+    started from that field.  If J_emu > J_check + rel_energy_gap *
+    |J_check|, the solve continues (up to ``max_its`` iterations with early
+    stopping) and its field, masked, replaces the emulated one.  With
+    check_its = 0 the check only records J_emu.  Records
+    state.COST_EMULATED (a float64 0-dim device tensor), state.COST_CHECK
+    (float) and state.fallback_count.
     """
     fb = cfg.processes.iceflow.emulator.fallback
     state.fallback_calls = getattr(state, "fallback_calls", 0) + 1
@@ -229,13 +265,26 @@ def fallback_check(cfg, state):
     if (state.fallback_calls - 1) % max(1, int(fb.check_freq)) != 0:
         return
     op = energy_operator(cfg, state)
-    j_emu_dev = op.energy(state.U, state.V)[0, 3]
+    full = getattr(state, "iceflow_unmasked", None)
+    if full is None:
+        Y = getattr(state, "iceflow_Y", None)
+        if Y is None:
+            full = (state.U, state.V)
+        else:
+            nz = int(cfg.processes.iceflow.numerics.Nz)
+            full = (Y[0, ..., :nz].permute(2, 0, 1).contiguous(), Y[0, ..., nz:].permute(2, 0, 1).contiguous())
+    ice = (state.thk > 0).to(full[0].dtype)[None, None]
+    near_ice = torch.nn.functional.max_pool2d(ice, 3, stride=1, padding=1)[0, 0] > 0
+    zero = torch.zeros((), dtype=full[0].dtype, device=full[0].device)
+    U0 = torch.where(near_ice, full[0], zero).contiguous()
+    V0 = torch.where(near_ice, full[1], zero).contiguous()
+    j_emu_dev = op.energy(U0, V0)[0, 3]
     state.COST_EMULATED = j_emu_dev  # float64 0-dim device tensor
     k = int(fb.check_its)
     if k <= 0:
         return  # (nothing to decide: the host does not wait for the GPU)
+    res = slv.solve(op, U0, V0, k, state.optimizer, early_stopping=False)
     j_emu = float(j_emu_dev)
-    res = slv.solve(op, state.U, state.V, k, state.optimizer, early_stopping=False)
     j_chk = float(min(np.min(res.costs), j_emu))
     state.COST_CHECK = j_chk
     if j_emu > j_chk + float(fb.rel_energy_gap) * abs(j_chk):

@@ -122,12 +122,16 @@ class PatchBatchBatcher:
     def next(self):
         parts = [patch_tensor(p, self.dev) for p in syn.patch_batch(self.step, n=self.batch)]
         self.step += 1
+        return self.pack(parts)
+
+    def pack(self, parts):
+        """(X, extents) sub-batches of a list of (h, w, 5) patches."""
         packer = Batcher.__new__(Batcher)
         packer.dev, packer.pixels, packer.padded = self.dev, 0, 0
         if self.buckets == 1:
             out = [packer._pack(parts)]
         else:
-            parts.sort(key=lambda p: p.shape[0] * p.shape[1])
+            parts = sorted(parts, key=lambda p: p.shape[0] * p.shape[1])
             cuts = np.linspace(0, len(parts), self.buckets + 1).round().astype(int)
             out = [packer._pack(parts[c0:c1]) for c0, c1 in zip(cuts[:-1], cuts[1:]) if c1 > c0]
         self.pixels += packer.pixels

@@ -206,17 +206,21 @@ def rel_err(U, V, Ur, Vr, mask):
     return float(torch.linalg.vector_norm(d * m) / torch.linalg.vector_norm(r * m))
 
 
-def converged_solution(op32, op64, U0, V0, n_flat=8000, n_decay=20000, n_refine=2000):
+def converged_solution(op32, op64, U0, V0, n_flat=8000, n_decay=20000, n_refine=2000, lr_end=1e-3,
+                       epsilon=1e-7):
     """Float64 minimiser of one field: float32 Adam from (U0, V0) at lr 10
-    for n_flat iterations, a geometric decay to 1e-3 over n_decay, then
-    float64 Adam from that point with the step decaying 1e-3 -> 1e-5.
+    for n_flat iterations, a geometric decay to ``lr_end`` over n_decay, then
+    float64 Adam from that point with the step decaying lr_end -> lr_end/100
+    (Adam's ``epsilon`` in both stages).
 
     Returns (U, V, COST, change) with ``change`` the relative difference the
-    float64 stage made (a measure of how converged the float32 stage was).
+    float64 stage made.  Fields whose velocities are weakly determined (thin
+    ice) leave runs of different schedules ~1e-3 apart at energies equal to
+    ~1e-7: compare two schedules to know how far a minimiser can be trusted.
     """
-    r = solve(op32, U0, V0, n_flat + n_decay, LegacyAdamState(lr=1.0), early_stopping=False,
-              lr_schedule=adam_schedule(10.0, n_flat, n_decay))
-    r2 = solve(op64, r.U.double(), r.V.double(), n_refine, LegacyAdamState(lr=1.0), early_stopping=False,
-               lr_schedule=adam_schedule(1e-3, 0, n_refine, lr_end=1e-5))
+    r = solve(op32, U0, V0, n_flat + n_decay, LegacyAdamState(lr=1.0, epsilon=epsilon), early_stopping=False,
+              lr_schedule=adam_schedule(10.0, n_flat, n_decay, lr_end))
+    r2 = solve(op64, r.U.double(), r.V.double(), n_refine, LegacyAdamState(lr=1.0, epsilon=epsilon),
+               early_stopping=False, lr_schedule=adam_schedule(lr_end, 0, n_refine, lr_end=lr_end / 100))
     change = rel_err(r.U, r.V, r2.U, r2.V, torch.ones_like(r.U[0], dtype=torch.bool))
     return r2.U, r2.V, float(r2.costs[-1]), change
