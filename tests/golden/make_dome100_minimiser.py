@@ -9,7 +9,9 @@ independently of that method, on the CPU, from the oracle's energy and its
 autograd gradient (oracle/energy.py, float64):
 
 1. L-BFGS from U = V = 0 (scipy) until the energy settles;
-2. cycles of L-BFGS in preconditioned variables x = x_c + L^-T y, where
+2. three damped Newton steps (backtracking line search), the Newton system
+   solved by conjugate gradients preconditioned by the column blocks below;
+3. cycles of L-BFGS in preconditioned variables x = x_c + L^-T y, where
    L L^T is the block diagonal of the Hessian at the cycle's start point,
    one 20 x 20 block per node column (U and V at the Nz levels; the
    vertical coupling is the stiffest part of the problem).  The blocks come
@@ -18,8 +20,13 @@ autograd gradient (oracle/energy.py, float64):
 
 The file records the field (float64), its energy, the gradient norm, and
 the relative change of the field over each cycle's last 500 iterations
-(``tail_change``): the last one bounds how far the field can still be from
-the minimiser.  Run time: about 30 minutes on 8 cores.
+(``tail_change``).  The minimiser's velocities are weakly determined along
+some directions (mostly thin ice): fields that differ by ~1e-3 relative can
+differ in energy by only ~1e-7 relative, so the energy (the lowest found by
+any of the methods tried: this one, long decaying-step legacy Adam in float32
+and float64 with epsilon 1e-7 or 1e-16) is the certificate, and the stored
+field is a minimiser to ~1e-3 in velocity.  Run time: about 30 minutes on 8
+cores.
 
     python tests/golden/make_dome100_minimiser.py [--cycles N]
 """
@@ -84,6 +91,63 @@ class Problem:
         return float((a - b)[:, :, self.ice].norm() / b[:, :, self.ice].norm())
 
 
+def block_inverse(P, x):
+    """Inverse of the column blocks of the Hessian at x (unknowns without any
+    energy dependence get identity blocks) and the mask of the others."""
+    B = P.column_blocks(x)
+    act = torch.diagonal(B, dim1=-2, dim2=-1) > 0
+    B = B * (act[..., :, None] & act[..., None, :]).double() + torch.diag_embed((~act).double())
+    return torch.linalg.inv(B), act.reshape(P.ny, P.nx, 2, NZ).permute(2, 3, 0, 1)
+
+
+def newton_step(P, x, log, cg_its=500, tol=1e-2):
+    """x + s p: p from preconditioned CG on H p = -g (to tol relative
+    residual), s halved from 1 until the energy decreases enough."""
+    ny, nx = P.ny, P.nx
+    Binv, act = block_inverse(P, x)
+    xr = x.detach().requires_grad_(True)
+    c0 = P.J(xr)
+    (g,) = torch.autograd.grad(c0, xr, create_graph=True)
+
+    def hv(v):
+        (h,) = torch.autograd.grad(g, xr, grad_outputs=v, retain_graph=True)
+        return h * act
+
+    def prec(r):
+        rr = r.permute(2, 3, 0, 1).reshape(ny, nx, 2 * NZ, 1)
+        return (Binv @ rr).reshape(ny, nx, 2, NZ).permute(2, 3, 0, 1) * act
+
+    b = -(g.detach() * act)
+    p = torch.zeros_like(x)
+    r = b.clone()
+    z = prec(r)
+    d = z.clone()
+    rz = float((r * z).sum())
+    bn = float(b.norm())
+    for k in range(cg_its):
+        Hd = hv(d)
+        dHd = float((d * Hd).sum())
+        if dHd <= 0:
+            break
+        a = rz / dHd
+        p += a * d
+        r -= a * Hd
+        if float(r.norm()) < tol * bn:
+            break
+        z = prec(r)
+        rz2 = float((r * z).sum())
+        d = z + (rz2 / rz) * d
+        rz = rz2
+    c = float(c0.detach())
+    gp = float((g.detach() * p).sum())
+    s = 1.0
+    while s > 1e-8 and float(P.J(x + s * p).detach()) > c + 1e-4 * s * gp:
+        s *= 0.5
+    xn = (x + s * p).detach()
+    log(f"{k + 1} CG iterations, step {s}, J {float(P.J(xn).detach()):.15f}, change {P.rel_change(xn, x):.2e}")
+    return xn
+
+
 def lbfgs(fun, y0, iters, log, every=500):
     """scipy L-BFGS for ``iters`` iterations; returns (y, [iterates every ``every``])."""
     snaps = []
@@ -102,8 +166,9 @@ def lbfgs(fun, y0, iters, log, every=500):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--plain-its", type=int, default=3000)
-    ap.add_argument("--cycles", type=int, default=4)
-    ap.add_argument("--cycle-its", type=int, default=3000)
+    ap.add_argument("--newton", type=int, default=3)
+    ap.add_argument("--cycles", type=int, default=2)
+    ap.add_argument("--cycle-its", type=int, default=2000)
     args = ap.parse_args()
     torch.set_num_threads(min(8, os.cpu_count() or 1))
     P = Problem()
@@ -119,6 +184,8 @@ def main():
     y, _ = lbfgs(plain, x.numpy().ravel(), args.plain_its,
                  lambda k: print(f"plain L-BFGS {k} its ({time.time() - t0:.0f} s)", flush=True))
     x = torch.as_tensor(y).view(shape).clone()
+    for it in range(args.newton):
+        x = newton_step(P, x, lambda m: print(f"Newton {it}: {m} ({time.time() - t0:.0f} s)", flush=True))
     tail = []
     for cyc in range(args.cycles):
         B = P.column_blocks(x)
