@@ -62,9 +62,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_domes1000.json")
 # schedule family searched for both paths: Adam (beta1, beta2), initial step,
 # fraction of the iterations at it, step at the end of a geometric decay
 BETAS = ((0.9, 0.999), (0.95, 0.99), (0.95, 0.95))
-LR0S = (100.0, 300.0, 1000.0)
+LR0S = (100.0, 300.0)
 FLATS = (0.0, 0.1, 0.3)
 ENDS = (10.0, 30.0, 60.0)
+# Adam's epsilon: the Keras default, and one below the per-node gradients of a
+# 1000 x 1000 field (~1e-8 .. 1e-6: the energy is a mean over 10^6 cells),
+# where the default turns Adam into plain gradient descent on small ones
+EPSS = (1e-7, 1e-9)
 TUNE_SEEDS = (21, 22)     # held-out domes1000 fields the schedules are chosen on
 TUNE_TARGET = 0.009       # every held-out field must reach this error
 REFINE_TOTALS = (100, 150, 200, 250, 300, 400, 500, 700, 1000, 1500, 2000, 3000)
@@ -158,14 +162,16 @@ def run_schedule(op, U0, V0, sch):
     from igm_amd.solver import LegacyAdamState, adam_schedule, solve
     n = sch["its"]
     n_flat = int(sch["flat"] * n)
-    return solve(op, U0, V0, n, LegacyAdamState(lr=sch["lr0"], beta1=sch["beta1"], beta2=sch["beta2"]),
+    return solve(op, U0, V0, n, LegacyAdamState(lr=sch["lr0"], beta1=sch["beta1"], beta2=sch["beta2"],
+                                                epsilon=sch["eps"]),
                  early_stopping=False, lr_schedule=adam_schedule(sch["lr0"], n_flat, n - n_flat, sch["lr_end"]))
 
 
 def schedule_search(fields, start, totals, target):
-    """Shortest schedule of the family {Adam (beta1, beta2) in BETAS; lr0 in
-    LR0S; a fraction in FLATS at lr0, then a geometric decay to a step in
-    ENDS} that brings EVERY field of ``fields`` (from ``start(field)``) to an
+    """Shortest schedule of the family {Adam (beta1, beta2) in BETAS and
+    epsilon in EPSS; lr0 in LR0S; a fraction in FLATS at lr0, then a
+    geometric decay to a step in ENDS} that brings EVERY field of ``fields``
+    (from ``start(field)``) to an
     error <= target: the first total of ``totals`` whose best schedule does
     (best = lowest worst-case error), then bisected to 2 iterations against
     the total before it.  Returns the schedule dict with the worst-case
@@ -173,22 +179,24 @@ def schedule_search(fields, start, totals, target):
 
     def trial(total):
         best = None
-        for b1, b2 in BETAS:
-            for lr0 in LR0S:
-                for ff in FLATS:
-                    for lr_end in ENDS:
-                        sch = {"its": total, "lr0": lr0, "flat": ff, "lr_end": lr_end, "beta1": b1, "beta2": b2}
-                        errs = []
-                        for fd in fields:
-                            U0, V0 = start(fd)
-                            r = run_schedule(fd.op32, U0, V0, sch)
-                            errs.append(fd.err(r.U, r.V))
-                            if best is not None and max(errs) >= best["err"]:
-                                break
-                        if best is None or max(errs) < best["err"]:
-                            best = dict(sch, err=max(errs), errs=errs)
+        for eps in EPSS:
+            for b1, b2 in BETAS:
+                for lr0 in LR0S:
+                    for ff in FLATS:
+                        for lr_end in ENDS:
+                            sch = {"its": total, "lr0": lr0, "flat": ff, "lr_end": lr_end, "beta1": b1, "beta2": b2,
+                                   "eps": eps}
+                            errs = []
+                            for fd in fields:
+                                U0, V0 = start(fd)
+                                r = run_schedule(fd.op32, U0, V0, sch)
+                                errs.append(fd.err(r.U, r.V))
+                                if best is not None and max(errs) >= best["err"]:
+                                    break
+                            if best is None or max(errs) < best["err"]:
+                                best = dict(sch, err=max(errs), errs=errs)
         log(f"[bench]   {total} its: worst held-out err {best['err']:.4f} (lr0 {best['lr0']}, flat {best['flat']}, "
-            f"end {best['lr_end']}, betas {best['beta1']}, {best['beta2']})")
+            f"end {best['lr_end']}, betas {best['beta1']}, {best['beta2']}, eps {best['eps']:g})")
         return best
 
     lo, best = None, None
@@ -449,9 +457,9 @@ def main():
         f"{', '.join(f'{f.err(f.Ue, f.Ve):.4f}' for f in tune_f)}); choosing the polish on seeds {TUNE_SEEDS}")
     pol = schedule_search(tune_f, lambda f: (f.Ue, f.Ve), REFINE_TOTALS, TUNE_TARGET)
     rf = ice.emulator.refine
-    rf.its, rf.lr, rf.flat_frac, rf.lr_end, rf.beta1, rf.beta2 = (int(pol["its"]), float(pol["lr0"]),
-                                                                   float(pol["flat"]), float(pol["lr_end"]),
-                                                                   float(pol["beta1"]), float(pol["beta2"]))
+    rf.its, rf.lr, rf.flat_frac, rf.lr_end, rf.beta1, rf.beta2, rf.epsilon = (
+        int(pol["its"]), float(pol["lr0"]), float(pol["flat"]), float(pol["lr_end"]), float(pol["beta1"]),
+        float(pol["beta2"]), float(pol["eps"]))
     fb = ice.emulator.fallback
 
     def emulator_step():
@@ -592,7 +600,8 @@ def main():
 
     def sched_out(s):
         return {"its": int(s["its"]), "lr0": s["lr0"], "flat_frac": s["flat"], "lr_end": s["lr_end"],
-                "adam_betas": [s["beta1"], s["beta2"]], "held_out_errs": [round(e, 5) for e in s["errs"]]}
+                "adam_betas": [s["beta1"], s["beta2"]], "adam_epsilon": s["eps"],
+                "held_out_errs": [round(e, 5) for e in s["errs"]]}
 
     out = {
         "metric": METRIC,

@@ -106,10 +106,10 @@ ICEFLOW = {
         },
         # build-specific polish of every emulated field: `its` plain-solver
         # iterations warm-started from it, step lr for flat_frac of them,
-        # then a geometric decay to lr_end (0 its: off); beta1, beta2 of its
-        # Adam (the Keras defaults unless set)
+        # then a geometric decay to lr_end (0 its: off); beta1, beta2 and
+        # epsilon of its Adam (the Keras defaults unless set)
         "refine": {"its": 0, "lr": 3.0, "flat_frac": 0.3, "lr_end": 1.0e-3, "beta1": 0.9,
-                   "beta2": 0.999},
+                   "beta2": 0.999, "epsilon": 1.0e-7},
     },
 }
 

@@ -213,7 +213,7 @@ def refine_emulated(cfg, state):
     """Polish the emulated field: ``refine.its`` plain-solver iterations
     warm-started from state.U, state.V with a fresh legacy-Adam state, step
     ``refine.lr`` for the first ``flat_frac`` of them, then a geometric decay
-    to ``lr_end``, with Adam's ``beta1``, ``beta2``; the outputs are
+    to ``lr_end``, with Adam's ``beta1``, ``beta2`` and ``epsilon``; the outputs are
     rebuilt from the polished field.  A network whose error is a few percent
     reaches the solver's accuracy in a small fraction of the iterations a
     solve from zero needs."""
@@ -226,7 +226,8 @@ def refine_emulated(cfg, state):
     op = energy_operator(cfg, state)
     lr = float(rf["lr"])
     n_flat = int(float(rf["flat_frac"]) * n)
-    opt = slv.LegacyAdamState(lr=lr, beta1=float(rf["beta1"]), beta2=float(rf["beta2"]))
+    opt = slv.LegacyAdamState(lr=lr, beta1=float(rf["beta1"]), beta2=float(rf["beta2"]),
+                              epsilon=float(rf["epsilon"]))
     res = slv.solve(op, state.U, state.V, n, opt, early_stopping=False,
                     lr_schedule=slv.adam_schedule(lr, n_flat, n - n_flat, float(rf["lr_end"])))
     out = velocity_outputs(res.U, res.V, state.thk, state.vert_weight, int(ice.numerics.Nz))
