@@ -160,6 +160,47 @@ def test_head_store_paths_and_extents(dev, cout, offset):
     assert torch.all(out[offset:offset + n].reshape(B, H, W, cout)[1, 13:] == 0.0)
 
 
+@pytest.mark.parametrize("cout,offset", [(20, 0), (20, 1), (18, 0), (10, 0), (10, 3), (7, 0), (32, 2)])
+def test_head_bwd_load_paths_and_extents(dev, cout, offset):
+    """head_bwd reads dY with 16-B loads when cout % 4 == 0 and dY is 16-B
+    aligned, 4-B loads otherwise (cout 18, 10, 7, or dY offset by `offset`
+    floats).  dz16 = bf16((dY W^T) * LeakyReLU'(a16)) and dY_bf16 = bf16(dY)
+    zero-padded to 32 channels; both are zero outside a sample's extent."""
+    print(f"case head_bwd cout={cout} offset={offset}", flush=True)
+    B, H, W = 3, 19, 47
+    g = torch.Generator().manual_seed(100 + cout * 10 + offset)
+    dY = torch.randn((B, H, W, cout), generator=g)
+    a = bf(torch.randn((B, H, W, 32), generator=g))
+    w = torch.randn((32, cout), generator=g) * 0.2  # HWIO [32 in][cout]
+    ext = torch.tensor([[H, W], [11, 30], [H, 1]], dtype=torch.int32)
+    n = B * H * W * cout
+    buf = torch.zeros((n + offset + 4,), dtype=torch.float32, device=dev)
+    dYd = buf[offset:offset + n]
+    dYd.copy_(dY.reshape(-1).to(dev))
+    ad = a.to(torch.bfloat16).to(dev)
+    wd = w.to(dev).contiguous()
+    dz = torch.full((B, H, W, 32), 3.0, dtype=torch.bfloat16, device=dev)
+    dYb = torch.full((B, H, W, 32), 3.0, dtype=torch.bfloat16, device=dev)
+    ed = ext.to(dev)
+    L.call("igm_conv1x1_head_bwd", L.ptr(dYd), L.ptr(ad), L.ptr(wd), L.ptr(ed), B, H, W, cout,
+           L.ptr(dz), L.ptr(dYb), L.stream_ptr())
+    torch.cuda.synchronize()
+    slope = torch.where(a > 0, 1.0, 0.01).double()
+    ref_dz = (dY.double() @ w.double().T) * slope
+    ref_dYb = torch.zeros((B, H, W, 32))
+    ref_dYb[..., :cout] = dY.to(torch.bfloat16).float()
+    for b in range(B):
+        keep = torch.zeros((H, W), dtype=torch.bool)
+        keep[:int(ext[b, 0]), :int(ext[b, 1])] = True
+        ref_dz[b][~keep] = 0.0
+        ref_dYb[b][~keep] = 0.0
+    got_dz = dz.float().cpu()
+    assert rel(got_dz, ref_dz) <= 4e-3
+    # the bf16 copy of dY is the round-to-nearest-even of each value, exactly
+    assert torch.equal(dYb.float().cpu(), ref_dYb)
+    assert torch.all(got_dz[1, 11:] == 0.0) and torch.all(got_dz[2, :, 1:] == 0.0)
+
+
 def test_conv_in_wgrad(dev):
     B, H, W = 2, 30, 300
     g = torch.Generator().manual_seed(5)
