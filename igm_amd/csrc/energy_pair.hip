@@ -14,11 +14,12 @@
 // At step c:
 //   stage A computes cell row c at the input velocity U0, which completes
 //     the gradient of node row c; the first Adam update turns node row c
-//     into U1 and its moments into m1, both kept in registers;
-//   stage B computes cell row c-1 at U1 (node rows c-1 and c, in registers),
-//     which completes node row c-1 for the second iteration; the second Adam
-//     update writes U2 and the moments m2 of that row.
-// U1 and m1 never leave the registers.
+//     into U1 and its moments into m1, kept in registers and in LDS;
+//   stage B computes cell row c-1 at U1 (node row c-1 from LDS, row c in
+//     registers), which completes node row c-1 for the second iteration; the
+//     second Adam update writes U2 and the moments m2 of that row.
+// U1 and m1 never leave the CU.  The two stages run level by level, B one
+// level behind A, as two independent instruction streams.
 //
 // The price is a wider halo.  U1 is only exact where stage A's gradient is
 // complete, lanes 1..62 of the 64, so stage B's gradient is complete on lanes
@@ -33,7 +34,14 @@
 // one-iteration passes, bit for bit.
 #include "energy_pass.hpp"
 
+#include <type_traits>
+
 namespace igm {
+
+// step modes of energy_pair_kernel
+constexpr int kOneIter = 0;   // one iteration
+constexpr int kStageA = 1;    // two iterations, stage A only
+constexpr int kPairStep = 2;  // two iterations, stages A and B
 
 // Prologue of a device-driven solve pass (a.alpha_tab set), done by every
 // block.  The state *es_in is the one the previous pass started from, so the
@@ -153,12 +161,11 @@ energy_pair_kernel(const PassArgs<float> a) {
 #pragma unroll
       for (int p = 0; p < kPlanes; ++p) c[p] = bload(rC, co, p * ps, T());
     };
-    // moments of node row `row` (rows j0-1 .. j1 give U1; others read 0)
-    auto load_mom = [&](T (&m)[NZ][4], int row) {
+    // byte offset of the moments of node row `row`, level 0 (rows j0-1 .. j1
+    // give U1; the others read 0)
+    auto mom_off = [&](int row) -> uint32_t {
       const uint32_t o = node_off(row, row >= j0 - 1 && row <= j1);
-      const uint32_t om = o == kOOB ? kOOB : 4u * o;
-#pragma unroll
-      for (int k = 0; k < NZ; ++k) mload(rMi, om, k * mstep, m[k]);
+      return o == kOOB ? kOOB : 4u * o;
     };
 
     // upper-corner adjoint contributions of the previous cell row, per stage
@@ -169,121 +176,146 @@ energy_pair_kernel(const PassArgs<float> a) {
     }
 
     // Stage B's inputs from the previous step -- U1 and the moments m1 of
-    // node row c-1 -- wait in LDS rather than in registers, where they would
-    // be live across the whole of stage A.  Each wave has a private pair of
-    // slots that alternate with the row parity: step c writes row c to slot
-    // c & 1 after its first update, and its stage B reads row c-1 from the
-    // other slot.  LDS accesses of one wave complete in order, so no barrier
-    // is needed.
+    // node row c-1 -- wait in LDS.  Each wave has a private pair of slots
+    // that alternate with the row parity: step c writes row c to slot c & 1
+    // in its first update, and its stage B reads row c-1 from the other slot.
+    // LDS accesses of one wave complete in order, so no barrier is needed.
     float4 (&sm)[2][NZ][kWave] = lds_m[threadIdx.x >> 6];
     float2 (&sq)[2][NZ][kWave] = lds_q[threadIdx.x >> 6];
 
-    // One step c.  In registers on entry: U0 node rows c (a_lo) and c+1
-    // (a_hi) and the coefficients of cell rows c (k_cur) and c-1 (k_prv).
-    // The step issues the next step's loads first (vector memory returns in
-    // order, so the sweeps never wait on them): the moments of node row c
-    // (needed at the end of stage A), the coefficients of cell row c+1 and
-    // U0 of node row c+2.
-    auto step = [&](int c, const T (&a_lo_u)[NZ], const T (&a_lo_v)[NZ], const T (&a_hi_u)[NZ],
-                    const T (&a_hi_v)[NZ], T (&a_nx_u)[NZ], T (&a_nx_v)[NZ],
-                    const T (&k_cur)[kPlanes], T (&k_nx)[kPlanes], const T (&k_prv)[kPlanes]) {
-      T m[NZ][4];
-      load_mom(m, c);
+    // Moments of the node row of the step's first update, in registers.  The
+    // first update of level k consumes m[k] and reloads it with the next
+    // row's level k, so every level's moments arrive a step ahead.
+    T m[NZ][4];
+    {
+      const uint32_t om = mom_off(j0 - 2);
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) mload(rMi, om, k * mstep, m[k]);
+    }
+
+    // One step c; the mode is kOneIter (one iteration: the first update is
+    // the output), kStageA (two iterations, no stage B: the wave's first two
+    // steps, cell rows j0-2 and j0-1, would only feed node rows below its
+    // own) or kPairStep (stages A and B).  In registers on entry: U0 node rows
+    // c (a_lo) and c+1 (a_hi) and the coefficients of cell rows c (k_cur) and
+    // c-1 (k_prv).  The step first issues the next step's loads (vector memory
+    // returns in order, so the sweeps never wait on them): the coefficients
+    // of cell row c+1 and U0 of node row c+2.
+    //
+    // Stage A (cell row c at U0) and stage B (cell row c-1 at U1) run level
+    // by level, B one level behind: A's level k emits the gradient of node
+    // row c at level k-1, whose first update gives U1 of that level, which B's
+    // level k-1 reads.  The two sweeps are independent streams of
+    // instructions in one basic block, which the scheduler interleaves.
+    auto step = [&](auto mode, int c, const T (&a_lo_u)[NZ], const T (&a_lo_v)[NZ],
+                    const T (&a_hi_u)[NZ], const T (&a_hi_v)[NZ], T (&a_nx_u)[NZ],
+                    T (&a_nx_v)[NZ], const T (&k_cur)[kPlanes], T (&k_nx)[kPlanes],
+                    const T (&k_prv)[kPlanes]) {
+      constexpr int M = decltype(mode)::value;
       load_coef(k_nx, c + 1);
       load_row(a_nx_u, a_nx_v, c + 2);
       // keep the loads above ahead of the sweeps (the scheduler would sink them)
       __builtin_amdgcn_sched_barrier(0);
 
-      // stage A: cell row c at U0 -> gradient of node row c
-      T gu[NZ], gv[NZ];
-      const CellEnergy<T> eA = level_sweep<T, NZ>(
-          a, hh2, h05, h025, a_lo_u, a_lo_v, a_hi_u, a_hi_v, k_cur, cuA, cvA,
-          [&](int k, T g_u, T g_v) {
-            gu[k] = g_u;
-            gv[k] = g_v;
-          });
-      // keep the first update after the sweep (it waits for the moments)
-      __builtin_amdgcn_sched_barrier(0);
-      // first update: U1 and m1 of node row c, stashed for the next step
-      T q_u[NZ], q_v[NZ];
-      const int sn = c & 1;
-#pragma unroll
-      for (int k = 0; k < NZ; ++k) {
-        adam_step(m[k], gu[k], gv[k], a.omb1, a.omb2, alpha1, a.eps, a_lo_u[k], a_lo_v[k],
-                  q_u[k], q_v[k]);
-        sm[sn][k][lane] = make_float4(m[k][0], m[k][1], m[k][2], m[k][3]);
-        sq[sn][k][lane] = make_float2(q_u[k], q_v[k]);
-      }
-      if (!pair) {
-        // one iteration: U1 and m1 are the pass's output (owned node rows)
-        const uint32_t o1 = node_off(c, own && c >= j0 && c < j1);
-        const uint32_t om1 = o1 == kOOB ? kOOB : 4u * o1;
-#pragma unroll
-        for (int k = 0; k < NZ; ++k) {
+      const int sn = c & 1, sp = sn ^ 1;
+      const uint32_t om_next = mom_off(c + 1);
+      // one iteration: U1 and m1 of node row c are the output (owned rows)
+      const uint32_t o1 = M == kOneIter ? node_off(c, own && c >= j0 && c < j1) : kOOB;
+      const uint32_t om1 = o1 == kOOB ? kOOB : 4u * o1;
+      // stage B's output: U2 and m2 of node row c-1 (owned rows)
+      const bool emit2 = own && (c - 1 >= j0);
+      const uint32_t o2 = M == kPairStep ? node_off(c - 1, emit2) : kOOB;
+      const uint32_t om2 = o2 == kOOB ? kOOB : 4u * o2;
+
+      T q_u[NZ], q_v[NZ];  // U1 of node row c
+      T p_u[NZ], p_v[NZ];  // U1 of node row c-1 (stage B)
+      // first update of node row c at level k, from stage A's gradient
+      auto first = [&](int k, T g_u, T g_v) {
+        adam_step(m[k], g_u, g_v, a.omb1, a.omb2, alpha1, a.eps, a_lo_u[k], a_lo_v[k], q_u[k],
+                  q_v[k]);
+        if constexpr (M == kOneIter) {
           bstore(q_u[k], rUo, o1, k * kstep);
           bstore(q_v[k], rVo, o1, k * kstep);
           mstore(m[k], rMo, om1, k * mstep);
+        } else {
+          sm[sn][k][lane] = make_float4(m[k][0], m[k][1], m[k][2], m[k][3]);
+          sq[sn][k][lane] = make_float2(q_u[k], q_v[k]);
+        }
+        mload(rMi, om_next, k * mstep, m[k]);
+      };
+      // second update of node row c-1 at level k, from stage B's gradient
+      auto second = [&](int k, T g_u, T g_v) {
+        const float4 t = sm[sp][k][lane];
+        T mk[4] = {t.x, t.y, t.z, t.w};
+        T un_, vn_;
+        adam_step(mk, g_u, g_v, a.omb1, a.omb2, alpha2, a.eps, p_u[k], p_v[k], un_, vn_);
+        bstore(un_, rUo, o2, k * kstep);
+        bstore(vn_, rVo, o2, k * kstep);
+        mstore(mk, rMo, om2, k * mstep);
+      };
+      auto stage_b_level = [&](LevelSweep<T, NZ>& sb, int k) {
+        const float2 t = sq[sp][k][lane];
+        p_u[k] = t.x;
+        p_v[k] = t.y;
+        sb.level(k, p_u[k], p_v[k], q_u[k], q_v[k], cuB, cvB, second);
+      };
+
+      LevelSweep<T, NZ> sa(a, hh2, h05, h025, k_cur);
+      LevelSweep<T, NZ> sb(a, hh2, h05, h025, k_prv);
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) {
+        sa.level(k, a_lo_u[k], a_lo_v[k], a_hi_u[k], a_hi_v[k], cuA, cvA, first);
+        if constexpr (M == kPairStep) {
+          if (k >= 1) stage_b_level(sb, k - 1);
         }
       }
-
+      const CellEnergy<T> eA = sa.finish(cuA, cvA, first);
       if (own && c >= j0 && c < j1) {
         e1_sh += (double)eA.sh;
         e1_sl += (double)eA.sl;
         e1_gr += (double)eA.gr;
       }
-
-      // stage B: cell row c-1 at U1 (node rows c-1 from LDS and c) ->
-      // gradient of node row c-1, second update, stored (owned node rows
-      // only).  In the wave's first two steps (cell rows j0-3 and j0-2) it
-      // would only feed node rows below the wave's own, so the wave skips it
-      // (a wave-uniform branch).
-      if (pair && c >= j0) {
-        const int sp = sn ^ 1;
-        T p_u[NZ], p_v[NZ];
-#pragma unroll
-        for (int k = 0; k < NZ; ++k) {
-          const float2 t = sq[sp][k][lane];
-          p_u[k] = t.x;
-          p_v[k] = t.y;
-        }
-        const bool emit = own && (c - 1 >= j0);
-        const uint32_t on = node_off(c - 1, emit);
-        const uint32_t om = on == kOOB ? kOOB : 4u * on;
-        const CellEnergy<T> eB = level_sweep<T, NZ>(
-            a, hh2, h05, h025, p_u, p_v, q_u, q_v, k_prv, cuB, cvB, [&](int k, T g_u, T g_v) {
-              const float4 t = sm[sp][k][lane];
-              T mk[4] = {t.x, t.y, t.z, t.w};
-              T un_, vn_;
-              adam_step(mk, g_u, g_v, a.omb1, a.omb2, alpha2, a.eps, p_u[k], p_v[k], un_, vn_);
-              bstore(un_, rUo, on, k * kstep);
-              bstore(vn_, rVo, on, k * kstep);
-              mstore(mk, rMo, om, k * mstep);
-            });
-        if (emit) {
+      if constexpr (M == kPairStep) {
+        stage_b_level(sb, NZ - 1);
+        const CellEnergy<T> eB = sb.finish(cuB, cvB, second);
+        if (emit2) {
           e2_sh += (double)eB.sh;
           e2_sl += (double)eB.sl;
           e2_gr += (double)eB.gr;
         }
       }
     };
+    using OneIter = std::integral_constant<int, kOneIter>;
+    using StageA = std::integral_constant<int, kStageA>;
+    using PairStep = std::integral_constant<int, kPairStep>;
 
-    // U0 rows and coefficient rows in rotation with period 3 (the loop is
-    // unrolled three times, so the rotation costs no moves).  The set that
-    // the first unrolled step would read in stage B starts at zero (that
-    // stage B is skipped at c = j0-2).
+    // U0 rows and coefficient rows in rotation with period 3 (the loops are
+    // unrolled three times, so the rotation costs no moves).  A wave has at
+    // least two node rows (j1 >= j0 + 1), so its first two steps, c = j0-2 and
+    // j0-1, always run; in a two-iteration pass they have no stage B.
     T A0u[NZ], A0v[NZ], A1u[NZ], A1v[NZ], A2u[NZ], A2v[NZ];
     T K0[kPlanes], K1[kPlanes], K2[kPlanes];
-#pragma unroll
-    for (int p = 0; p < kPlanes; ++p) K2[p] = T(0);
     load_row(A0u, A0v, j0 - 2);
     load_row(A1u, A1v, j0 - 1);
     load_coef(K0, j0 - 2);
-    for (int c = j0 - 2; c <= j1; c += 3) {
-      step(c, A0u, A0v, A1u, A1v, A2u, A2v, K0, K1, K2);
-      if (c + 1 > j1) break;
-      step(c + 1, A1u, A1v, A2u, A2v, A0u, A0v, K1, K2, K0);
-      if (c + 2 > j1) break;
-      step(c + 2, A2u, A2v, A0u, A0v, A1u, A1v, K2, K0, K1);
+    if (!pair) {
+      for (int c = j0 - 2; c <= j1; c += 3) {
+        step(OneIter(), c, A0u, A0v, A1u, A1v, A2u, A2v, K0, K1, K2);
+        if (c + 1 > j1) break;
+        step(OneIter(), c + 1, A1u, A1v, A2u, A2v, A0u, A0v, K1, K2, K0);
+        if (c + 2 > j1) break;
+        step(OneIter(), c + 2, A2u, A2v, A0u, A0v, A1u, A1v, K2, K0, K1);
+      }
+    } else {
+      step(StageA(), j0 - 2, A0u, A0v, A1u, A1v, A2u, A2v, K0, K1, K2);
+      step(StageA(), j0 - 1, A1u, A1v, A2u, A2v, A0u, A0v, K1, K2, K0);
+      for (int c = j0; c <= j1; c += 3) {
+        step(PairStep(), c, A2u, A2v, A0u, A0v, A1u, A1v, K2, K0, K1);
+        if (c + 1 > j1) break;
+        step(PairStep(), c + 1, A0u, A0v, A1u, A1v, A2u, A2v, K0, K1, K2);
+        if (c + 2 > j1) break;
+        step(PairStep(), c + 2, A1u, A1v, A2u, A2v, A0u, A0v, K1, K2, K0);
+      }
     }
   }
 

@@ -238,8 +238,8 @@ struct CellEnergy {
 // Level sweep of the cell whose lower-left node is (jc, ic): node rows jc
 // (up, vp) and jc+1 (uc, vc) of all levels and the cell's coefficients cc,
 // all in registers.  The right neighbour's nodes come from lane l+1 (DPP).
-// For every level k, once its adjoint is complete, calls emit(k, g_u, g_v)
-// with dJ/dU and dJ/dV of node (jc, ic): this cell's lower-left corner, the
+// For every level k, once its adjoint is complete, emit(k, g_u, g_v) gets
+// dJ/dU and dJ/dV of node (jc, ic): this cell's lower-left corner, the
 // lower-right corner of the cell of lane l-1 (DPP), and the carry (cu, cv)
 // of cell row jc-1, which the call replaces by this row's upper-corner
 // contributions.  Layer l = k-1 is processed once level k is known, so only
@@ -247,42 +247,56 @@ struct CellEnergy {
 // units of 1/(4 dx) (ux = 4 dx dU/dx at the level), the vertical ones
 // likewise (q holds damp*dx/Hc); hh2 = 1/(4 dx)^2 restores the strain rate
 // where it is needed, h05 = hh2/2, h025 = hh2/4.
-template <typename T, int NZ, typename Emit>
-__device__ __forceinline__ CellEnergy<T> level_sweep(const PassArgs<T>& a, T hh2, T h05, T h025,
-                                                    const T (&up)[NZ], const T (&vp)[NZ],
-                                                    const T (&uc)[NZ], const T (&vc)[NZ],
-                                                    const T (&cc)[kPlanes], T (&cu)[NZ],
-                                                    T (&cv)[NZ], Emit&& emit) {
-  // Multiply-adds are fused within one expression only, never across
-  // statements: the fusion then follows the source, not the surrounding
-  // kernel, and every kernel that runs this sweep rounds the same way.
-#pragma clang fp contract(on)
-  const T q = cc[0], w = cc[1], gx = cc[2], gy = cc[3], csl = cc[4], bx = cc[5], by = cc[6];
-  const T whh = w * hh2;
+//
+// LevelSweep runs the sweep one level at a time: level(k, ...) for k = 0 ..
+// NZ-1 in order, then finish().  level(k) emits level k-1 (k >= 1) and
+// finish() emits level NZ-1, so a caller can interleave two sweeps whose
+// inputs depend on each other's outputs (the two-iteration pass).
+template <typename T, int NZ>
+struct LevelSweep {
+  const PassArgs<T>& a;
+  const T hh2, h05, h025;
+  const T q, w, gx, gy, csl, bx, by, whh;
+  T p_ux = 0, p_uy = 0, p_vx = 0, p_vy = 0, p_su = 0, p_sv = 0;  // level k-1
+  T g_ux = 0, g_uy = 0, g_vx = 0, g_vy = 0, g_su = 0, g_sv = 0;  // its adjoint
+  T c_sh = 0, c_sl = 0, wsu = 0, wsv = 0;
+
+  __device__ __forceinline__ LevelSweep(const PassArgs<T>& a_, T hh2_, T h05_, T h025_,
+                                        const T (&cc)[kPlanes])
+      : a(a_), hh2(hh2_), h05(h05_), h025(h025_), q(cc[0]), w(cc[1]), gx(cc[2]), gy(cc[3]),
+        csl(cc[4]), bx(cc[5]), by(cc[6]), whh(cc[1] * hh2_) {}
+
   // Adjoint scatter of one level to the 4 corner nodes.  With A = gs - gx
   // and B = gs + gx, the corners get A -+ gy (left) and B -+ gy (right); the
   // right ones arrive from lane l-1: node row jc receives X - Z, node row
   // jc+1 (carried) X + Z, X = A + B(l-1), Z = gy + gy(l-1).
-  auto scatter = [&](T& carry, T gx_, T gy_, T gs_) -> T {
+  __device__ __forceinline__ static T scatter(T& carry, T gx_, T gy_, T gs_) {
+#pragma clang fp contract(on)
     const T A = gs_ - gx_, B = gs_ + gx_;
     const T X = A + lane_from_left(B);
     const T Z = gy_ + lane_from_left(gy_);
     const T g = (carry - Z) + X;
     carry = X + Z;
     return g;
-  };
-  auto out = [&](int k, T gux, T guy, T gsu, T gvx, T gvy, T gsv) {
-    const T g_u = scatter(cu[k], gux, guy, gsu);
-    const T g_v = scatter(cv[k], gvx, gvy, gsv);
+  }
+
+  template <typename Emit>
+  __device__ __forceinline__ void out(int k, T (&cu)[NZ], T (&cv)[NZ], Emit&& emit) {
+    const T g_u = scatter(cu[k], g_ux, g_uy, g_su);
+    const T g_v = scatter(cv[k], g_vx, g_vy, g_sv);
     emit(k, g_u, g_v);
-  };
-  T p_ux = 0, p_uy = 0, p_vx = 0, p_vy = 0, p_su = 0, p_sv = 0;  // level k-1
-  T g_ux = 0, g_uy = 0, g_vx = 0, g_vy = 0, g_su = 0, g_sv = 0;  // its adjoint
-  T c_sh = 0, c_sl = 0, wsu = 0, wsv = 0;
-#pragma unroll
-  for (int k = 0; k < NZ; ++k) {
-    const T su = up[k] + uc[k], du = uc[k] - up[k];
-    const T sv = vp[k] + vc[k], dv = vc[k] - vp[k];
+  }
+
+  // Level k from the nodes (upk, vpk) of row jc and (uck, vck) of row jc+1.
+  // Multiply-adds are fused within one expression only, never across
+  // statements: the fusion then follows the source, not the surrounding
+  // kernel, and every kernel that runs this sweep rounds the same way.
+  template <typename Emit>
+  __device__ __forceinline__ void level(int k, T upk, T vpk, T uck, T vck, T (&cu)[NZ],
+                                        T (&cv)[NZ], Emit&& emit) {
+#pragma clang fp contract(on)
+    const T su = upk + uck, du = uck - upk;
+    const T sv = vpk + vck, dv = vck - vpk;
     const T su_r = lane_from_right(su), du_r = lane_from_right(du);
     const T sv_r = lane_from_right(sv), dv_r = lane_from_right(dv);
     const T ux = su_r - su, uy = du + du_r;
@@ -343,13 +357,31 @@ __device__ __forceinline__ CellEnergy<T> level_sweep(const PassArgs<T>& a, T hh2
       n_su += gzu;
       g_sv -= gzv;
       n_sv += gzv;
-      out(k - 1, g_ux, g_uy, g_su, g_vx, g_vy, g_sv);
+      out(k - 1, cu, cv, emit);
     }
     p_ux = ux; p_uy = uy; p_vx = vx; p_vy = vy; p_su = sU; p_sv = sV;
     g_ux = n_ux; g_uy = n_uy; g_vx = n_vx; g_vy = n_vy; g_su = n_su; g_sv = n_sv;
   }
-  out(NZ - 1, g_ux, g_uy, g_su, g_vx, g_vy, g_sv);
-  return {w * c_sh, c_sl, gx * wsu + gy * wsv};
+
+  // Emits level NZ-1 and returns the cell's (shear, sliding, gravity) energy.
+  template <typename Emit>
+  __device__ __forceinline__ CellEnergy<T> finish(T (&cu)[NZ], T (&cv)[NZ], Emit&& emit) {
+#pragma clang fp contract(on)
+    out(NZ - 1, cu, cv, emit);
+    return {w * c_sh, c_sl, gx * wsu + gy * wsv};
+  }
+};
+
+template <typename T, int NZ, typename Emit>
+__device__ __forceinline__ CellEnergy<T> level_sweep(const PassArgs<T>& a, T hh2, T h05, T h025,
+                                                    const T (&up)[NZ], const T (&vp)[NZ],
+                                                    const T (&uc)[NZ], const T (&vc)[NZ],
+                                                    const T (&cc)[kPlanes], T (&cu)[NZ],
+                                                    T (&cv)[NZ], Emit&& emit) {
+  LevelSweep<T, NZ> sw(a, hh2, h05, h025, cc);
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) sw.level(k, up[k], vp[k], uc[k], vc[k], cu, cv, emit);
+  return sw.finish(cu, cv, emit);
 }
 
 // One legacy-Adam step (solve.py:84, TF's legacy Adam with alpha carrying
