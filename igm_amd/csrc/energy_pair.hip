@@ -139,10 +139,16 @@ energy_pair_kernel(const PassArgs<float> a) {
     const T hh2 = hh * hh;
     const T h05 = T(0.5) * hh2, h025 = T(0.25) * hh2;
 
-    // byte offset of node (row, ic), level 0 -- or kOOB outside the grid
+    // byte offset of node (row, ic), level 0 -- or kOOB outside the grid.
+    // The conditions here and below combine with & rather than &&, and both
+    // offsets are computed, so that the compiler selects instead of
+    // branching: a branch would split a step into blocks, and at their joins
+    // the memory-counter waits become conservative (they then also wait for
+    // the stores stage B issued last).
     auto node_off = [&](int row, bool extra) -> uint32_t {
-      const bool ok = extra && col_ok && row >= 0 && row < a.Ny;
-      return ok ? (uint32_t)(row * a.Nx + ic) * ES : kOOB;
+      const bool ok = extra & col_ok & (row >= 0) & (row < a.Ny);
+      const uint32_t o = (uint32_t)(row * a.Nx + ic) * ES;
+      return ok ? o : kOOB;
     };
     // node row `row` of U0 and V0 (stage A reads rows j0-2 .. j1+1)
     auto load_row = [&](T (&u)[NZ], T (&v)[NZ], int row) {
@@ -156,16 +162,18 @@ energy_pair_kernel(const PassArgs<float> a) {
     // coefficient planes of cell row jc (stage A reads rows j0-2 .. j1)
     const uint32_t ps = (uint32_t)a.plane * ES;
     auto load_coef = [&](T (&c)[kPlanes], int jc) {
-      const uint32_t co =
-          (col_ok && jc >= 0 && jc < a.Ny && jc <= j1) ? (uint32_t)(jc * a.Nx + ic) * ES : kOOB;
+      const bool ok = col_ok & (jc >= 0) & (jc < a.Ny) & (jc <= j1);
+      const uint32_t o = (uint32_t)(jc * a.Nx + ic) * ES;
+      const uint32_t co = ok ? o : kOOB;
 #pragma unroll
       for (int p = 0; p < kPlanes; ++p) c[p] = bload(rC, co, p * ps, T());
     };
     // byte offset of the moments of node row `row`, level 0 (rows j0-1 .. j1
     // give U1; the others read 0)
     auto mom_off = [&](int row) -> uint32_t {
-      const uint32_t o = node_off(row, row >= j0 - 1 && row <= j1);
-      return o == kOOB ? kOOB : 4u * o;
+      const bool ok = col_ok & (row >= 0) & (row < a.Ny) & (row >= j0 - 1) & (row <= j1);
+      const uint32_t o = (uint32_t)(row * a.Nx + ic) * (4u * ES);
+      return ok ? o : kOOB;
     };
 
     // upper-corner adjoint contributions of the previous cell row, per stage
@@ -220,10 +228,11 @@ energy_pair_kernel(const PassArgs<float> a) {
       const int sn = c & 1, sp = sn ^ 1;
       const uint32_t om_next = mom_off(c + 1);
       // one iteration: U1 and m1 of node row c are the output (owned rows)
-      const uint32_t o1 = M == kOneIter ? node_off(c, own && c >= j0 && c < j1) : kOOB;
+      const bool own_c = own & (c >= j0) & (c < j1);  // node and cell row c are the wave's
+      const uint32_t o1 = M == kOneIter ? node_off(c, own_c) : kOOB;
       const uint32_t om1 = o1 == kOOB ? kOOB : 4u * o1;
       // stage B's output: U2 and m2 of node row c-1 (owned rows)
-      const bool emit2 = own && (c - 1 >= j0);
+      const bool emit2 = own & (c - 1 >= j0);
       const uint32_t o2 = M == kPairStep ? node_off(c - 1, emit2) : kOOB;
       const uint32_t om2 = o2 == kOOB ? kOOB : 4u * o2;
 
@@ -270,19 +279,15 @@ energy_pair_kernel(const PassArgs<float> a) {
         }
       }
       const CellEnergy<T> eA = sa.finish(cuA, cvA, first);
-      if (own && c >= j0 && c < j1) {
-        e1_sh += (double)eA.sh;
-        e1_sl += (double)eA.sl;
-        e1_gr += (double)eA.gr;
-      }
+      e1_sh += own_c ? (double)eA.sh : 0.0;
+      e1_sl += own_c ? (double)eA.sl : 0.0;
+      e1_gr += own_c ? (double)eA.gr : 0.0;
       if constexpr (M == kPairStep) {
         stage_b_level(sb, NZ - 1);
         const CellEnergy<T> eB = sb.finish(cuB, cvB, second);
-        if (emit2) {
-          e2_sh += (double)eB.sh;
-          e2_sl += (double)eB.sl;
-          e2_gr += (double)eB.gr;
-        }
+        e2_sh += emit2 ? (double)eB.sh : 0.0;
+        e2_sl += emit2 ? (double)eB.sl : 0.0;
+        e2_gr += emit2 ? (double)eB.gr : 0.0;
       }
     };
     using OneIter = std::integral_constant<int, kOneIter>;
