@@ -94,9 +94,18 @@ class EnergyOperator:
         self.hscale = torch.empty((B,), dtype=dtype, device=dev)
         nws = L.load().igm_energy_workspace_doubles(B, Ny, Nx)
         self.partials = torch.zeros((max(1, nws),), dtype=torch.float64, device=dev)
+        self._partials_b = None
         self.out = torch.zeros((B, 4), dtype=torch.float64, device=dev)
         L.call(f"igm_energy_coeffs_{self.sfx}", *ptrs, fb, fr, fc, L.ptr(extents), B, Ny, Nx,
                C.addressof(self.phys), L.ptr(self.coef), L.ptr(self.hscale), L.stream_ptr())
+
+    @property
+    def partials_b(self):
+        """A second partials buffer (a pass's fused finalize reads the previous
+        pass's partials while it writes its own)."""
+        if self._partials_b is None:
+            self._partials_b = torch.zeros_like(self.partials)
+        return self._partials_b
 
     # -- raw passes ---------------------------------------------------------
     def run_pass(self, mode, U, V=None, gU=None, gV=None, Uo=None, Vo=None, moments=None, adam=None,

@@ -28,7 +28,7 @@ import torch
 from . import config as cfgmod
 from . import emulator as emu
 from . import solver as slv
-from .energy import EnergyOperator
+from .energy import EnergyOperator, physics_struct
 from .fields import as_device, velocity_outputs
 from .vert_disc import compute_levels, compute_zeta_dzeta, define_vertical_weight
 
@@ -85,11 +85,27 @@ def initialize_iceflow_solver(cfg, state):
 
 
 def energy_operator(cfg, state, dtype=torch.float32):
+    """The energy operator of the state's fields.  The float32 one is kept on
+    the state and reused while its inputs are unchanged: the same field
+    tensors (address and version -- torch counts every in-place write), the
+    same vertical grid and physics.  The cache entry holds the field tensors,
+    so no other tensor can take their memory, and address, meanwhile."""
     fields = {f: getattr(state, f) for f in FIELDIN}
     if dtype != torch.float32:
         fields = {k: v.to(dtype) for k, v in fields.items()}
-    return EnergyOperator(np.asarray(state.dzeta), cfg.processes.iceflow.physics, fields=fields,
-                          dtype=dtype)
+        return EnergyOperator(np.asarray(state.dzeta), cfg.processes.iceflow.physics,
+                              fields=fields, dtype=dtype)
+    dz = np.asarray(state.dzeta, dtype=np.float64)
+    phys = cfg.processes.iceflow.physics
+    key = (tuple((f.data_ptr(), f._version, tuple(f.shape), f.dtype, f.device)
+                 for f in fields.values()),
+           dz.tobytes(), bytes(physics_struct(phys)))
+    cached = getattr(state, "_energy_op", None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    op = EnergyOperator(dz, phys, fields=fields, dtype=dtype)
+    state._energy_op = (key, op, tuple(fields.values()))
+    return op
 
 
 def solve_iceflow(cfg, state, U, V, nbitmax=None):

@@ -69,6 +69,39 @@ class SolveResult:
         return self._costs
 
 
+# Pinned host buffers for the small uploads of a solve, in a ring: a buffer
+# is refilled only once the copy out of it queued last time has run (its
+# event), which with several buffers has long happened.
+_RING = [[None, None] for _ in range(4)]
+_ring_next = [0]
+
+
+def upload(chunks, dev):
+    """One host-to-device copy of the concatenated byte strings ``chunks``,
+    queued on the current stream from pinned memory (the host does not wait
+    for the GPU work before it).  Returns the device uint8 tensor."""
+    n = sum(len(c) for c in chunks)
+    slot = _RING[_ring_next[0]]
+    _ring_next[0] = (_ring_next[0] + 1) % len(_RING)
+    buf, ev = slot
+    if ev is not None:
+        ev.synchronize()
+    if buf is None or buf.numel() < n:
+        buf = torch.empty(max(n, 1 << 14), dtype=torch.uint8).pin_memory()
+        slot[0] = buf
+    a = buf.numpy()
+    o = 0
+    for c in chunks:
+        a[o:o + len(c)] = np.frombuffer(c, dtype=np.uint8)
+        o += len(c)
+    d = torch.empty(max(n, 1), dtype=torch.uint8, device=dev)
+    d[:n].copy_(buf[:n], non_blocking=True)
+    ev = torch.cuda.Event()
+    ev.record()
+    slot[1] = ev
+    return d
+
+
 def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early_stopping=True,
           relative_min_delta=0.0002, patience=10, check_every=16, lr_schedule=None, pairs=True):
     """Minimise the energy of ``op`` from (U0, V0) (planar (Nz, Ny, Nx)).
@@ -87,8 +120,24 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     dt = op.dtype
     assert U0.dtype == dt and U0.shape == V0.shape and U0.dim() == 3
     dev = U0.device
-    Ub = [U0.detach().clone().contiguous(), torch.empty_like(U0)]
-    Vb = [V0.detach().clone().contiguous(), torch.empty_like(V0)]
+    U0, V0 = U0.detach().contiguous(), V0.detach().contiguous()
+    # Pass 0 reads (U0, V0) as given and writes buffer 0; pass k >= 1 reads
+    # the buffer pass k-1 wrote and writes the other one.  The input is never
+    # written, so it needs no copy.
+    Ub = [torch.empty_like(U0), torch.empty_like(U0)]
+    Vb = [torch.empty_like(V0), torch.empty_like(V0)]
+
+    def src(k):
+        return (U0, V0) if k == 0 else (Ub[(k - 1) % 2], Vb[(k - 1) % 2])
+
+    def dst(k):
+        return Ub[k % 2], Vb[k % 2]
+
+    def field_after(p):
+        """The field once p passes have applied their update (a copy of the
+        input for p = 0)."""
+        return (U0.clone(), V0.clone()) if p == 0 else dst(p - 1)
+
     # Adam slots, one (m_U, v_U, m_V, v_V) record per node and level
     mom = torch.zeros(tuple(U0.shape) + (4,), dtype=dt, device=dev)
     es_host = L.IgmEarlyStop(best_loss=0.0, relative_min_delta=relative_min_delta, has_best=0,
@@ -101,19 +150,20 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     # own partials to P[i % 2].  The state of the last launched pass is
     # E[n % 2], where a separate finalize then accounts for iteration n - 1.
     es_size = C.sizeof(L.IgmEarlyStop)
-    # (host-to-device copies from pinned memory are queued on the stream and
-    # do not make the host wait for the GPU work before them)
-    es_dev = torch.frombuffer(bytearray(bytes(es_host) * 2), dtype=torch.uint8).pin_memory().to(
-        dev, non_blocking=True)
-    E = [es_dev.data_ptr(), es_dev.data_ptr() + es_size]
-    P = [op.partials, torch.zeros_like(op.partials)]
-    hist = torch.zeros(max(1, nbitmax), dtype=torch.float64, device=dev)
-    adam = L.IgmAdam(alpha=0.0, beta1=optimizer.beta1, beta2=optimizer.beta2,
-                     epsilon=optimizer.epsilon)
     t0 = optimizer.iterations
     # the Adam step of every iteration, as the legacy optimizer computes it
     lrs = [lr_schedule(i) if lr_schedule is not None else optimizer.lr for i in range(nbitmax)]
     alphas = optimizer.alphas(t0 + 1, lrs, dt)
+    # the two states and the step table in one host-to-device copy (es_size
+    # is a multiple of 8, so the table is aligned)
+    up = upload([bytes(es_host) * 2, np.ascontiguousarray(alphas).tobytes()], dev)
+    es_dev = up[:2 * es_size]
+    E = [es_dev.data_ptr(), es_dev.data_ptr() + es_size]
+    P = [op.partials, op.partials_b]
+    # every COST this returns is written by a pass or a finalize
+    hist = torch.empty(max(1, nbitmax), dtype=torch.float64, device=dev)
+    adam = L.IgmAdam(alpha=0.0, beta1=optimizer.beta1, beta2=optimizer.beta2,
+                     epsilon=optimizer.epsilon)
 
     def result(last):
         raw = bytes(es_dev.cpu().numpy().tobytes())
@@ -130,14 +180,13 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
         # launching once the solve has ended.  A last launch accounts the
         # final pass; the field is then in buffer passes % 2.
         M = [mom, torch.empty_like(mom)]
-        atab = torch.from_numpy(np.ascontiguousarray(alphas, dtype=np.float32)).pin_memory().to(
-            dev, non_blocking=True)
+        assert dt == torch.float32
+        atab = up[2 * es_size:].view(torch.float32)
 
         def launch(k, finalize_only=False):
-            op.run_solve_pass(Ub[k % 2], Vb[k % 2], Ub[(k + 1) % 2], Vb[(k + 1) % 2], M[k % 2],
-                              M[(k + 1) % 2], adam, atab, nbitmax, P[k % 2],
-                              P[(k - 1) % 2] if k > 0 else None, E[k % 2], E[(k + 1) % 2], hist,
-                              finalize_only)
+            op.run_solve_pass(*src(k), *dst(k), M[k % 2], M[(k + 1) % 2], adam, atab, nbitmax,
+                              P[k % 2], P[(k - 1) % 2] if k > 0 else None, E[k % 2],
+                              E[(k + 1) % 2], hist, finalize_only)
 
         k = 0
         if not early_stopping:
@@ -149,7 +198,7 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
                 launch(j)
             launch(k, finalize_only=True)
             optimizer.iterations = t0 + nbitmax
-            return SolveResult(Ub[k % 2], Vb[k % 2], hist[:nbitmax], nbitmax, False)
+            return SolveResult(*field_after(k), hist[:nbitmax], nbitmax, False)
         else:
             while k < nbitmax:
                 launch(k)
@@ -162,16 +211,14 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
         st = result((k + 1) % 2)
         n = int(st.iters_done)
         optimizer.iterations = t0 + n
-        r = int(st.passes) % 2
-        return SolveResult(Ub[r], Vb[r], hist[:n], n, bool(st.stopped))
+        return SolveResult(*field_after(int(st.passes)), hist[:n], n, bool(st.stopped))
 
     n_launched = 0
     for i in range(nbitmax):
         adam.alpha = float(alphas[i])
-        src, dst = i % 2, (i + 1) % 2
         prev = None if i == 0 else (P[(i - 1) % 2], E[i % 2], E[(i - 1) % 2], hist, i - 1)
-        op.run_pass(2, Ub[src], Vb[src], Uo=Ub[dst], Vo=Vb[dst], moments=mom, adam=adam,
-                    partials=P[i % 2], prev=prev)
+        (Us, Vs), (Ud, Vd) = src(i), dst(i)
+        op.run_pass(2, Us, Vs, Uo=Ud, Vo=Vd, moments=mom, adam=adam, partials=P[i % 2], prev=prev)
         n_launched = i + 1
         if early_stopping and i > 0 and (i + 1) % check_every == 0:
             o = ((i - 1) % 2) * es_size + L.EARLYSTOP_STOPPED_OFFSET
@@ -185,9 +232,8 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     st = result(last)
     n = int(st.iters_done)
     optimizer.iterations = t0 + n
-    res = n % 2
     costs = hist[:n].cpu().numpy()
-    return SolveResult(Ub[res], Vb[res], costs, n, bool(st.stopped))
+    return SolveResult(*field_after(n), costs, n, bool(st.stopped))
 
 
 def adam_schedule(lr0, n_flat, n_decay, lr_end=1e-3):
