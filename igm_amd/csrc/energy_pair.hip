@@ -29,7 +29,7 @@
 // neighbouring waves read moment rows that this pass writes, the moments are
 // read from one buffer and written to another.
 //
-// Both stages run the level sweep of the one-iteration pass (level_sweep)
+// Both stages run the level sweep of the one-iteration pass (LevelSweep)
 // with the same operands, so a pair pass gives the same results as two
 // one-iteration passes, bit for bit.
 #include "energy_pass.hpp"
