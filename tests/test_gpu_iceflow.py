@@ -146,6 +146,44 @@ def test_refine_polishes_with_its_schedule_and_adam_betas(dev):
     assert state.COST_REFINED < float(op.energy(U0, V0)[0, 3])
 
 
+def test_energy_operator_cache_follows_its_inputs(dev):
+    """iceflow.energy_operator reuses the float32 operator while the fields
+    are the same tensors at the same version, and builds a new one after an
+    in-place write or a rebinding of a field, or another physics value; a
+    reused operator gives the energies of a freshly built one."""
+    from igm_amd.energy import EnergyOperator
+    cfg = config.default_cfg()
+    g = syn.dome100()
+    state = State()
+    for k in ("thk", "usurf", "dX"):
+        setattr(state, k, g[k])
+    state.it = -1
+    iceflow.initialize(cfg, state)
+    U, V = state.U.contiguous(), state.V.contiguous()
+
+    def fresh():
+        fields = {k: getattr(state, k) for k in iceflow.FIELDIN}
+        return EnergyOperator(np.asarray(state.dzeta), cfg.processes.iceflow.physics, fields=fields)
+
+    op1 = iceflow.energy_operator(cfg, state)
+    assert iceflow.energy_operator(cfg, state) is op1
+    assert torch.equal(op1.energy(U, V), fresh().energy(U, V))
+    state.thk.mul_(1.1)  # in place: the version changes
+    op2 = iceflow.energy_operator(cfg, state)
+    assert op2 is not op1
+    assert torch.equal(op2.energy(U, V), fresh().energy(U, V))
+    assert not torch.equal(op2.energy(U, V), op1.energy(U, V))
+    state.usurf = state.usurf + 5.0  # rebound
+    op3 = iceflow.energy_operator(cfg, state)
+    assert op3 is not op2
+    assert torch.equal(op3.energy(U, V), fresh().energy(U, V))
+    cfg.processes.iceflow.physics.exp_glen = 4.0
+    op4 = iceflow.energy_operator(cfg, state)
+    assert op4 is not op3
+    assert torch.equal(op4.energy(U, V), fresh().energy(U, V))
+    assert iceflow.energy_operator(cfg, state) is op4
+
+
 def test_fallback_replaces_a_failed_emulated_field(dev):
     """Energy-gap fallback: an emulator with zeroed weights predicts U = V = 0,
     whose energy lies far above what a few solver iterations reach, so the
