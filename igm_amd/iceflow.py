@@ -254,6 +254,21 @@ def refine_emulated(cfg, state):
 
 # ----------------------------------------------------------- fallback --
 
+def _near_ice(state, dtype):
+    """Nodes with ice in one of their cells: the 3 x 3 neighbourhood maximum of
+    thk > 0.  Kept on the state while thk is the same tensor at the same
+    version (the entry holds the tensor, so its address stays its own)."""
+    thk = state.thk
+    key = (thk.data_ptr(), thk._version, tuple(thk.shape), dtype)
+    cached = getattr(state, "_near_ice", None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    ice = (thk > 0).to(dtype)[None, None]
+    near = torch.nn.functional.max_pool2d(ice, 3, stride=1, padding=1)[0, 0] > 0
+    state._near_ice = (key, near, thk)
+    return near
+
+
 def fallback_check(cfg, state):
     """Energy-gap fallback of the emulated field to the plain solver.
 
@@ -290,17 +305,20 @@ def fallback_check(cfg, state):
         else:
             nz = int(cfg.processes.iceflow.numerics.Nz)
             full = (Y[0, ..., :nz].permute(2, 0, 1).contiguous(), Y[0, ..., nz:].permute(2, 0, 1).contiguous())
-    ice = (state.thk > 0).to(full[0].dtype)[None, None]
-    near_ice = torch.nn.functional.max_pool2d(ice, 3, stride=1, padding=1)[0, 0] > 0
+    near_ice = _near_ice(state, full[0].dtype)
     zero = torch.zeros((), dtype=full[0].dtype, device=full[0].device)
     U0 = torch.where(near_ice, full[0], zero).contiguous()
     V0 = torch.where(near_ice, full[1], zero).contiguous()
-    j_emu_dev = op.energy(U0, V0)[0, 3]
-    state.COST_EMULATED = j_emu_dev  # float64 0-dim device tensor
     k = int(fb.check_its)
     if k <= 0:
-        return  # (nothing to decide: the host does not wait for the GPU)
+        # nothing to decide: record J_emu (the host does not wait for the GPU)
+        state.COST_EMULATED = op.energy(U0, V0)[0, 3]  # float64 0-dim device tensor
+        return
     res = slv.solve(op, U0, V0, k, state.optimizer, early_stopping=False)
+    # the COST of the first iteration is the energy of the field it starts
+    # from: J_emu (float64 0-dim device tensor)
+    j_emu_dev = res.costs_device[0]
+    state.COST_EMULATED = j_emu_dev
     j_emu = float(j_emu_dev)
     j_chk = float(min(np.min(res.costs), j_emu))
     state.COST_CHECK = j_chk
