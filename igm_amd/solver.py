@@ -138,8 +138,9 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
         input for p = 0)."""
         return (U0.clone(), V0.clone()) if p == 0 else dst(p - 1)
 
-    # Adam slots, one (m_U, v_U, m_V, v_V) record per node and level
-    mom = torch.zeros(tuple(U0.shape) + (4,), dtype=dt, device=dev)
+    # Adam slots, one (m_U, v_U, m_V, v_V) record per node and level; a
+    # fresh optimizer starts from zero moments
+    mshape = tuple(U0.shape) + (4,)
     es_host = L.IgmEarlyStop(best_loss=0.0, relative_min_delta=relative_min_delta, has_best=0,
                              wait=0, patience=patience if early_stopping else 2**30, stopped=0,
                              stop_iter=-1, iters_done=0)
@@ -179,14 +180,16 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
         # it, the host reads the state every check_every launches and stops
         # launching once the solve has ended.  A last launch accounts the
         # final pass; the field is then in buffer passes % 2.
-        M = [mom, torch.empty_like(mom)]
+        # launch 0 reads no moments (the pass takes them as zero), so neither
+        # buffer needs clearing
+        M = [torch.empty(mshape, dtype=dt, device=dev), torch.empty(mshape, dtype=dt, device=dev)]
         assert dt == torch.float32
         atab = up[2 * es_size:].view(torch.float32)
 
         def launch(k, finalize_only=False):
-            op.run_solve_pass(*src(k), *dst(k), M[k % 2], M[(k + 1) % 2], adam, atab, nbitmax,
-                              P[k % 2], P[(k - 1) % 2] if k > 0 else None, E[k % 2],
-                              E[(k + 1) % 2], hist, finalize_only)
+            op.run_solve_pass(*src(k), *dst(k), M[k % 2] if k > 0 else None, M[(k + 1) % 2], adam,
+                              atab, nbitmax, P[k % 2], P[(k - 1) % 2] if k > 0 else None,
+                              E[k % 2], E[(k + 1) % 2], hist, finalize_only)
 
         k = 0
         if not early_stopping:
@@ -213,6 +216,7 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
         optimizer.iterations = t0 + n
         return SolveResult(*field_after(int(st.passes)), hist[:n], n, bool(st.stopped))
 
+    mom = torch.zeros(mshape, dtype=dt, device=dev)  # updated in place by every pass
     n_launched = 0
     for i in range(nbitmax):
         adam.alpha = float(alphas[i])

@@ -102,6 +102,32 @@ def test_pair_equals_two_single_passes(dev, ny, nx, nz):
     assert np.all(np.abs(hp - hs) <= 1e-12 * np.abs(hs)), (hp, hs)
 
 
+@pytest.mark.parametrize("ny,nx,nz", [(100, 100, 10), (37, 131, 5), (5, 7, 4)])
+def test_pair_without_input_moments_is_a_fresh_optimizer(dev, ny, nx, nz):
+    """moments_in = NULL stands for zero moments: the same U, V, moments and
+    COSTs, bit for bit, as a pass reading a zeroed buffer."""
+    g = small_glacier(ny, nx, seed=ny + 2 * nx)
+    U, V = smooth_velocity(nz, ny, nx, seed=nz + 3 * ny)
+    op = gpu_op(g, nz, dev)
+    U0, V0 = to_dev(U, dev), to_dev(V, dev)
+    st = LegacyAdamState(lr=3.0)
+    a1 = L.IgmAdam(alpha=st.alpha(1), beta1=0.9, beta2=0.999, epsilon=1e-7)
+    a2 = L.IgmAdam(alpha=st.alpha(2), beta1=0.9, beta2=0.999, epsilon=1e-7)
+    outs = []
+    for zero_in in (torch.zeros((nz, ny, nx, 4), device=dev), None):
+        mo = torch.full((nz, ny, nx, 4), float("nan"), device=dev)
+        Uo, Vo = torch.full_like(U0, float("nan")), torch.full_like(V0, float("nan"))
+        part = torch.zeros_like(op.partials)
+        hist = torch.zeros(2, dtype=torch.float64, device=dev)
+        es = es_device(dev)
+        op.run_pass2(U0, V0, Uo, Vo, zero_in, mo, a1, a2, part)
+        op.finalize_pair(es=es.data_ptr(), cost_hist=hist, it=0, partials=part)
+        outs.append((Uo, Vo, mo, hist))
+    torch.cuda.synchronize()
+    for name, a, b in zip(("U", "V", "moments", "COST"), outs[0], outs[1]):
+        assert first_mismatch(a, b) is None, (name, first_mismatch(a, b))
+
+
 def test_pair_errors(dev):
     g = small_glacier(20, 30, seed=1)
     op = gpu_op(g, 10, dev)
