@@ -61,9 +61,9 @@ FLOP_PER_PIXEL = 280640.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_domes1000.json")
 # schedule family searched for both paths: Adam (beta1, beta2), initial step,
 # fraction of the iterations at it, step at the end of a geometric decay
-BETAS = ((0.95, 0.99), (0.95, 0.98), (0.95, 0.95))
-LR0S = (100.0, 150.0, 300.0)
-FLATS = (0.1, 0.3, 0.5)
+BETAS = ((0.95, 0.99), (0.95, 0.98), (0.95, 0.95), (0.93, 0.95))
+LR0S = (100.0, 120.0, 150.0, 300.0)
+FLATS = (0.1, 0.2, 0.3, 0.5)
 ENDS = (10.0, 20.0, 30.0, 60.0)
 # Adam's epsilon: the Keras default, and one below the per-node gradients of a
 # 1000 x 1000 field (~1e-8 .. 1e-6: the energy is a mean over 10^6 cells),
