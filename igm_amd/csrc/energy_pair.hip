@@ -134,9 +134,10 @@ energy_pair_kernel(const PassArgs<float> a) {
     const rsrc_t rC = make_rsrc(a.coef, a.coef_bytes);
     // no input moments (a fresh optimizer): a resource of size 0, whose every
     // read is out of range and returns 0 -- the zero moments, with no bytes
-    // moved
+    // moved; no output moments (the last pass of a solve): every store is
+    // out of range and dropped
     const rsrc_t rMi = make_rsrc(a.mom, a.mom != nullptr ? a.mom_bytes : 0u);
-    const rsrc_t rMo = make_rsrc(a.mom_out, a.mom_bytes);
+    const rsrc_t rMo = make_rsrc(a.mom_out, a.mom_out != nullptr ? a.mom_bytes : 0u);
     const uint32_t kstep = (uint32_t)a.plane * ES;  // bytes between levels
     const uint32_t mstep = 4u * kstep;              // moments: 4 slots per node and level
     const T hh = a.hscale[0];
@@ -359,10 +360,10 @@ int pair_impl(int nz, int Ny, int Nx, const float* U, const float* V, const floa
               IgmEarlyStop* es_out, double* fin_out, double* cost_hist, int prev_iter,
               const float* alpha_tab, int nbitmax, int finalize_only, void* stream) {
   IGM_CHECK_ARG(U && V && coef && hscale && dzeta && phys && partials, "null pointer");
-  IGM_CHECK_ARG(Uo && Vo && mom_out && adam1 && adam2,
-                "needs Uout, Vout, the output moment buffer and both Adam steps");
+  IGM_CHECK_ARG(Uo && Vo && adam1 && adam2, "needs Uout, Vout and both Adam steps");
   IGM_CHECK_ARG(Uo != U && Vo != V && Uo != V && Vo != U, "Uout/Vout must not alias U/V");
-  IGM_CHECK_ARG(mom_out != mom_in, "the moments are read from one buffer and written to another");
+  IGM_CHECK_ARG(mom_out == nullptr || mom_out != mom_in,
+                "the moments are read from one buffer and written to another");
   IGM_CHECK_ARG(Ny >= 2 && Nx >= 2, "need a grid of at least 2 x 2");
   IGM_CHECK_ARG(nz >= 2 && nz <= IGM_MAX_NZ, "nz out of range");
   IGM_CHECK_ARG(adam1->beta1 == adam2->beta1 && adam1->beta2 == adam2->beta2 &&

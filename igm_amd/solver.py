@@ -186,10 +186,13 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
         assert dt == torch.float32
         atab = up[2 * es_size:].view(torch.float32)
 
-        def launch(k, finalize_only=False):
-            op.run_solve_pass(*src(k), *dst(k), M[k % 2] if k > 0 else None, M[(k + 1) % 2], adam,
-                              atab, nbitmax, P[k % 2], P[(k - 1) % 2] if k > 0 else None,
-                              E[k % 2], E[(k + 1) % 2], hist, finalize_only)
+        def launch(k, finalize_only=False, last=False):
+            # launch 0 reads no moments (zero); the last launch of a solve
+            # whose length is known writes none (nothing reads them)
+            op.run_solve_pass(*src(k), *dst(k), M[k % 2] if k > 0 else None,
+                              None if last else M[(k + 1) % 2], adam, atab, nbitmax, P[k % 2],
+                              P[(k - 1) % 2] if k > 0 else None, E[k % 2], E[(k + 1) % 2], hist,
+                              finalize_only)
 
         k = 0
         if not early_stopping:
@@ -198,7 +201,7 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
             # k % 2; the host does not wait for the GPU
             k = (nbitmax + 1) // 2
             for j in range(k):
-                launch(j)
+                launch(j, last=j == k - 1)
             launch(k, finalize_only=True)
             optimizer.iterations = t0 + nbitmax
             return SolveResult(*field_after(k), hist[:nbitmax], nbitmax, False)

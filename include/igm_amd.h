@@ -150,7 +150,8 @@ int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int3
  * Writes Uout, Vout (must not alias U, V).  Moments [nz][Ny][Nx][4] are read
  * from moments_in and written to moments_out (another buffer: neighbouring
  * waves read rows this pass writes); moments_in = NULL stands for all-zero
- * moments (a fresh optimizer: nothing is read).  adam1, adam2: same beta1,
+ * moments (a fresh optimizer: nothing is read), moments_out = NULL drops the
+ * updated moments (the last pass of a solve).  adam1, adam2: same beta1,
  * beta2 and epsilon; alpha may differ.
  * partials: 6 doubles per block, the (shear, sliding, gravity) sums of
  * iteration i then i+1 (igm_energy_workspace_doubles() covers them).
@@ -179,8 +180,8 @@ int igm_energy_pass2_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, con
  * Launch k (from 0) reads U, V and moments from buffer k % 2 and writes
  * buffer (k + 1) % 2 of each pair, partials to a buffer of its own (two in
  * turn), es_in = state k % 2, es_out = state (k + 1) % 2; launch 0 gets no
- * prev_partials, and moments_in = NULL for zero moments (as
- * igm_energy_pass2_f32).  A last launch with finalize_only = 1 accounts the final
+ * prev_partials, and moments_in = NULL for zero moments; moments_out = NULL
+ * drops the moments (as igm_energy_pass2_f32).  A last launch with finalize_only = 1 accounts the final
  * pass; the result is then in buffer passes % 2 (IgmEarlyStop.passes).
  * alpha_tab: device array of the Adam step alpha of iterations 0..nbitmax-1;
  * adam: beta1, beta2 and epsilon.  Float32, planar (Nz, Ny, Nx), one field;

@@ -105,7 +105,8 @@ def test_pair_equals_two_single_passes(dev, ny, nx, nz):
 @pytest.mark.parametrize("ny,nx,nz", [(100, 100, 10), (37, 131, 5), (5, 7, 4)])
 def test_pair_without_input_moments_is_a_fresh_optimizer(dev, ny, nx, nz):
     """moments_in = NULL stands for zero moments: the same U, V, moments and
-    COSTs, bit for bit, as a pass reading a zeroed buffer."""
+    COSTs, bit for bit, as a pass reading a zeroed buffer; moments_out = NULL
+    drops the moments and keeps U, V and the COSTs."""
     g = small_glacier(ny, nx, seed=ny + 2 * nx)
     U, V = smooth_velocity(nz, ny, nx, seed=nz + 3 * ny)
     op = gpu_op(g, nz, dev)
@@ -123,8 +124,17 @@ def test_pair_without_input_moments_is_a_fresh_optimizer(dev, ny, nx, nz):
         op.run_pass2(U0, V0, Uo, Vo, zero_in, mo, a1, a2, part)
         op.finalize_pair(es=es.data_ptr(), cost_hist=hist, it=0, partials=part)
         outs.append((Uo, Vo, mo, hist))
+    # moments_out = NULL: the moments are dropped; U, V and the COSTs are kept
+    Uo, Vo = torch.full_like(U0, float("nan")), torch.full_like(V0, float("nan"))
+    part = torch.zeros_like(op.partials)
+    hist = torch.zeros(2, dtype=torch.float64, device=dev)
+    es = es_device(dev)
+    op.run_pass2(U0, V0, Uo, Vo, None, None, a1, a2, part)
+    op.finalize_pair(es=es.data_ptr(), cost_hist=hist, it=0, partials=part)
     torch.cuda.synchronize()
     for name, a, b in zip(("U", "V", "moments", "COST"), outs[0], outs[1]):
+        assert first_mismatch(a, b) is None, (name, first_mismatch(a, b))
+    for name, a, b in (("U", Uo, outs[0][0]), ("V", Vo, outs[0][1]), ("COST", hist, outs[0][3])):
         assert first_mismatch(a, b) is None, (name, first_mismatch(a, b))
 
 
