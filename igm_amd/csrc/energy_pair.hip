@@ -87,9 +87,18 @@ __device__ int solve_prologue(const PassArgs<float>& a, bool block0, int* it0) {
   return a.finalize_only ? 0 : pass_iterations(st, st.iters_done, a.nbitmax);
 }
 
-template <int NZ>
+// Velocity layouts of a pass's input and output (template bits): planar, U
+// and V in two (Nz, Ny, Nx) buffers, or interleaved, one (Nz, Ny, Nx, 2)
+// buffer holding (U, V) per node and level -- one 8-B access where planar
+// takes two 4-B ones.  A solve keeps its intermediate fields interleaved.
+constexpr int kInInterleaved = 1;
+constexpr int kOutInterleaved = 2;
+
+template <int NZ, int LAYOUT>
 __global__ void __launch_bounds__(kPassBlock)
 energy_pair_kernel(const PassArgs<float> a) {
+  constexpr bool kInIL = (LAYOUT & kInInterleaved) != 0;
+  constexpr bool kOutIL = (LAYOUT & kOutInterleaved) != 0;
   using T = float;
   constexpr uint32_t ES = sizeof(T);
   const int lane = threadIdx.x & (kWave - 1);
@@ -129,8 +138,12 @@ energy_pair_kernel(const PassArgs<float> a) {
     // node rows [j0, j1) of this wave: the Ny rows split evenly over nrowblk
     const int j0 = (int)(((int64_t)rblk * a.Ny) / a.nrowblk);
     const int j1 = (int)(((int64_t)(rblk + 1) * a.Ny) / a.nrowblk);
-    const rsrc_t rU = make_rsrc(a.U, a.vel_bytes), rV = make_rsrc(a.V, a.vel_bytes);
-    const rsrc_t rUo = make_rsrc(a.Uo, a.vel_bytes), rVo = make_rsrc(a.Vo, a.vel_bytes);
+    // (an interleaved buffer holds twice the bytes of a planar one; its V
+    // resource is unused)
+    const rsrc_t rU = make_rsrc(a.U, kInIL ? 2u * a.vel_bytes : a.vel_bytes);
+    const rsrc_t rV = make_rsrc(kInIL ? a.U : a.V, a.vel_bytes);
+    const rsrc_t rUo = make_rsrc(a.Uo, kOutIL ? 2u * a.vel_bytes : a.vel_bytes);
+    const rsrc_t rVo = make_rsrc(kOutIL ? a.Uo : a.Vo, a.vel_bytes);
     const rsrc_t rC = make_rsrc(a.coef, a.coef_bytes);
     // no input moments (a fresh optimizer): a resource of size 0, whose every
     // read is out of range and returns 0 -- the zero moments, with no bytes
@@ -160,8 +173,28 @@ energy_pair_kernel(const PassArgs<float> a) {
       const uint32_t o = node_off(row, row <= j1 + 1);
 #pragma unroll
       for (int k = 0; k < NZ; ++k) {
-        u[k] = bload(rU, o, k * kstep, T());
-        v[k] = bload(rV, o, k * kstep, T());
+        if constexpr (kInIL) {
+          const auto w = __builtin_amdgcn_raw_buffer_load_b64(rU, o == kOOB ? kOOB : 2u * o,
+                                                              2u * k * kstep, 0);
+          u[k] = __uint_as_float(w[0]);
+          v[k] = __uint_as_float(w[1]);
+        } else {
+          u[k] = bload(rU, o, k * kstep, T());
+          v[k] = bload(rV, o, k * kstep, T());
+        }
+      }
+    };
+    // (u, v) of node offset o (kOOB: dropped), level k, into the output
+    auto store_uv = [&](T u, T v, uint32_t o, int k) {
+      if constexpr (kOutIL) {
+        typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+        u2 w;
+        w[0] = __float_as_uint(u);
+        w[1] = __float_as_uint(v);
+        __builtin_amdgcn_raw_buffer_store_b64(w, rUo, o == kOOB ? kOOB : 2u * o, 2u * k * kstep, 0);
+      } else {
+        bstore(u, rUo, o, k * kstep);
+        bstore(v, rVo, o, k * kstep);
       }
     };
     // coefficient planes of cell row jc (stage A reads rows j0-2 .. j1)
@@ -248,8 +281,7 @@ energy_pair_kernel(const PassArgs<float> a) {
         adam_step(m[k], g_u, g_v, a.omb1, a.omb2, alpha1, a.eps, a_lo_u[k], a_lo_v[k], q_u[k],
                   q_v[k]);
         if constexpr (M == kOneIter) {
-          bstore(q_u[k], rUo, o1, k * kstep);
-          bstore(q_v[k], rVo, o1, k * kstep);
+          store_uv(q_u[k], q_v[k], o1, k);
           mstore(m[k], rMo, om1, k * mstep);
         } else {
           sm[sn][k][lane] = make_float4(m[k][0], m[k][1], m[k][2], m[k][3]);
@@ -263,8 +295,7 @@ energy_pair_kernel(const PassArgs<float> a) {
         T mk[4] = {t.x, t.y, t.z, t.w};
         T un_, vn_;
         adam_step(mk, g_u, g_v, a.omb1, a.omb2, alpha2, a.eps, p_u[k], p_v[k], un_, vn_);
-        bstore(un_, rUo, o2, k * kstep);
-        bstore(vn_, rVo, o2, k * kstep);
+        store_uv(un_, vn_, o2, k);
         mstore(mk, rMo, om2, k * mstep);
       };
       auto stage_b_level = [&](LevelSweep<T, NZ>& sb, int k) {
@@ -359,9 +390,12 @@ int pair_impl(int nz, int Ny, int Nx, const float* U, const float* V, const floa
               const double* prev_partials, int prev_count, const IgmEarlyStop* es_in,
               IgmEarlyStop* es_out, double* fin_out, double* cost_hist, int prev_iter,
               const float* alpha_tab, int nbitmax, int finalize_only, void* stream) {
-  IGM_CHECK_ARG(U && V && coef && hscale && dzeta && phys && partials, "null pointer");
-  IGM_CHECK_ARG(Uo && Vo && adam1 && adam2, "needs Uout, Vout and both Adam steps");
-  IGM_CHECK_ARG(Uo != U && Vo != V && Uo != V && Vo != U, "Uout/Vout must not alias U/V");
+  IGM_CHECK_ARG(U && coef && hscale && dzeta && phys && partials, "null pointer");
+  IGM_CHECK_ARG(Uo && adam1 && adam2, "needs Uout and both Adam steps");
+  // V (Vout) NULL: U (Uout) holds (U, V) interleaved, (Nz, Ny, Nx, 2)
+  const int layout = (V == nullptr ? kInInterleaved : 0) | (Vo == nullptr ? kOutInterleaved : 0);
+  IGM_CHECK_ARG(Uo != U && (V == nullptr || Uo != V) && (Vo == nullptr || (Vo != U && Vo != V)),
+                "Uout/Vout must not alias U/V");
   IGM_CHECK_ARG(mom_out == nullptr || mom_out != mom_in,
                 "the moments are read from one buffer and written to another");
   IGM_CHECK_ARG(Ny >= 2 && Nx >= 2, "need a grid of at least 2 x 2");
@@ -414,8 +448,15 @@ int pair_impl(int nz, int Ny, int Nx, const float* U, const float* V, const floa
   a.eps = float(adam1->epsilon);
   hipStream_t st = (hipStream_t)stream;
   switch (nz) {
-#define IGM_PAIR_CASE(N) \
-  case N: hipLaunchKernelGGL((energy_pair_kernel<N>), dim3(g.blocks), dim3(kPassBlock), 0, st, a); break;
+#define IGM_PAIR_CASE(N)                                                                                \
+  case N:                                                                                               \
+    switch (layout) {                                                                                   \
+      case 0: hipLaunchKernelGGL((energy_pair_kernel<N, 0>), dim3(g.blocks), dim3(kPassBlock), 0, st, a); break; \
+      case 1: hipLaunchKernelGGL((energy_pair_kernel<N, 1>), dim3(g.blocks), dim3(kPassBlock), 0, st, a); break; \
+      case 2: hipLaunchKernelGGL((energy_pair_kernel<N, 2>), dim3(g.blocks), dim3(kPassBlock), 0, st, a); break; \
+      default: hipLaunchKernelGGL((energy_pair_kernel<N, 3>), dim3(g.blocks), dim3(kPassBlock), 0, st, a); break; \
+    }                                                                                                   \
+    break;
     IGM_PAIR_CASE(2)
     IGM_PAIR_CASE(3)
     IGM_PAIR_CASE(4)

@@ -53,6 +53,16 @@ def velocity_strides(U, V=None, nz=None):
     return (U.data_ptr(), V.data_ptr(), U.stride(0), U.stride(1), U.stride(2), U.stride(3))
 
 
+def _check_pair_layout(op, U, V, Uo, Vo):
+    """Shapes of a two-iteration pass's velocity buffers: planar (Nz, Ny, Nx)
+    pairs, or with V (Vo) None an interleaved (Nz, Ny, Nx, 2) U (Uo)."""
+    assert op.pair_supported
+    for u, v in ((U, V), (Uo, Vo)):
+        shape = (op.nz, op.Ny, op.Nx) + (() if v is not None else (2,))
+        assert tuple(u.shape) == shape and u.is_contiguous(), (tuple(u.shape), shape)
+        assert v is None or (tuple(v.shape) == shape and v.is_contiguous())
+
+
 class EnergyOperator:
     """Coefficients + evaluation of the energy for a fixed set of input fields.
 
@@ -147,7 +157,7 @@ class EnergyOperator:
         written to ``mom_out``.  ``partials`` receives 6 doubles per block.
         ``prev``: (prev_partials, prev_count, es_in, es_out, cost_hist,
         prev_iter) -- the previous pass's finalize, done by this pass."""
-        assert self.pair_supported and U.dim() == 3 and U.is_contiguous() and V.is_contiguous()
+        _check_pair_layout(self, U, V, Uo, Vo)
         L.call("igm_energy_pass2_f32", self.nz, self.Ny, self.Nx, L.ptr(U), L.ptr(V),
                L.ptr(self.coef), L.ptr(self.hscale), C.addressof(self.dzeta), C.addressof(self.phys),
                L.ptr(Uo), L.ptr(Vo), L.ptr(mom_in), L.ptr(mom_out), C.addressof(adam1),
@@ -163,7 +173,7 @@ class EnergyOperator:
         EarlyStopping state allows.  ``alpha_tab``: float32 device tensor of
         the per-iteration Adam steps; ``es_in``, ``es_out``: device addresses
         of two IgmEarlyStop states."""
-        assert self.pair_supported and U.dim() == 3 and U.is_contiguous() and V.is_contiguous()
+        _check_pair_layout(self, U, V, Uo, Vo)
         L.call("igm_energy_solve_pass_f32", self.nz, self.Ny, self.Nx, L.ptr(U), L.ptr(V),
                L.ptr(self.coef), L.ptr(self.hscale), C.addressof(self.dzeta), C.addressof(self.phys),
                L.ptr(Uo), L.ptr(Vo), L.ptr(mom_in), L.ptr(mom_out), C.addressof(adam),

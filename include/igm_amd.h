@@ -147,7 +147,10 @@ int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int3
  * adam1), then again at the updated field (iteration i+1, step adam2), with
  * no EarlyStopping check between the two.  Float32, planar (Nz, Ny, Nx), one
  * field.  The result equals two igm_energy_pass_f32 mode-2 calls.
- * Writes Uout, Vout (must not alias U, V).  Moments [nz][Ny][Nx][4] are read
+ * Writes Uout, Vout (must not alias U, V).  V = NULL: U holds (U, V)
+ * interleaved, [nz][Ny][Nx][2]; Vout = NULL: Uout is written that way (one
+ * 8-B access per node and level instead of two 4-B ones; a solve keeps its
+ * intermediate fields so).  Moments [nz][Ny][Nx][4] are read
  * from moments_in and written to moments_out (another buffer: neighbouring
  * waves read rows this pass writes); moments_in = NULL stands for all-zero
  * moments (a fresh optimizer: nothing is read), moments_out = NULL drops the
@@ -181,7 +184,8 @@ int igm_energy_pass2_f32(int32_t nz, int32_t Ny, int32_t Nx, const float* U, con
  * buffer (k + 1) % 2 of each pair, partials to a buffer of its own (two in
  * turn), es_in = state k % 2, es_out = state (k + 1) % 2; launch 0 gets no
  * prev_partials, and moments_in = NULL for zero moments; moments_out = NULL
- * drops the moments (as igm_energy_pass2_f32).  A last launch with finalize_only = 1 accounts the final
+ * drops the moments, and V or Vout = NULL selects the interleaved layout (as
+ * igm_energy_pass2_f32).  A last launch with finalize_only = 1 accounts the final
  * pass; the result is then in buffer passes % 2 (IgmEarlyStop.passes).
  * alpha_tab: device array of the Adam step alpha of iterations 0..nbitmax-1;
  * adam: beta1, beta2 and epsilon.  Float32, planar (Nz, Ny, Nx), one field;

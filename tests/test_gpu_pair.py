@@ -138,6 +138,44 @@ def test_pair_without_input_moments_is_a_fresh_optimizer(dev, ny, nx, nz):
         assert first_mismatch(a, b) is None, (name, first_mismatch(a, b))
 
 
+@pytest.mark.parametrize("ny,nx,nz", [(100, 100, 10), (37, 131, 5), (5, 7, 4), (61, 60, 12)])
+def test_pair_interleaved_layouts(dev, ny, nx, nz):
+    """V (Vout) = NULL: U (Uout) holds (U, V) interleaved per node and level.
+    All four input/output layouts give the planar pass's U, V, moments and
+    COSTs bit for bit."""
+    g = small_glacier(ny, nx, seed=3 * ny + nx)
+    U, V = smooth_velocity(nz, ny, nx, seed=nz + 5 * ny)
+    op = gpu_op(g, nz, dev)
+    U0, V0 = to_dev(U, dev), to_dev(V, dev)
+    UV0 = torch.stack([U0, V0], dim=-1).contiguous()
+    mom0 = random_moments(nz, ny, nx, ny * nx, dev)
+    st = LegacyAdamState(lr=3.0)
+    a1 = L.IgmAdam(alpha=st.alpha(3), beta1=0.9, beta2=0.999, epsilon=1e-7)
+    a2 = L.IgmAdam(alpha=st.alpha(4), beta1=0.9, beta2=0.999, epsilon=1e-7)
+
+    def run(in_il, out_il):
+        mo = torch.full_like(mom0, float("nan"))
+        if out_il:
+            UVo = torch.full((nz, ny, nx, 2), float("nan"), device=dev)
+            dst = (UVo, None)
+        else:
+            dst = (torch.full_like(U0, float("nan")), torch.full_like(V0, float("nan")))
+        part = torch.zeros_like(op.partials)
+        hist = torch.zeros(2, dtype=torch.float64, device=dev)
+        es = es_device(dev)
+        op.run_pass2(*((UV0, None) if in_il else (U0, V0)), *dst, mom0, mo, a1, a2, part)
+        op.finalize_pair(es=es.data_ptr(), cost_hist=hist, it=0, partials=part)
+        torch.cuda.synchronize()
+        Uo, Vo = (dst[0][..., 0], dst[0][..., 1]) if out_il else dst
+        return Uo, Vo, mo, hist
+
+    ref = run(False, False)
+    for in_il, out_il in ((True, False), (False, True), (True, True)):
+        got = run(in_il, out_il)
+        for name, a, b in zip(("U", "V", "moments", "COST"), got, ref):
+            assert first_mismatch(a, b) is None, (in_il, out_il, name, first_mismatch(a, b))
+
+
 def test_pair_errors(dev):
     g = small_glacier(20, 30, seed=1)
     op = gpu_op(g, 10, dev)
