@@ -522,14 +522,15 @@ def main():
     kt = measure_kernels(cnn, op32, Ur.float().contiguous(), Vr.float().contiguous(), bench_f.f32,
                          state.vert_weight, nz)
     cells = float(thk.numel())
-    traffic = measured_traffic("energy_pair_kernel<10> (two fused iterations)")
+    traffic = measured_traffic("energy_pair_kernel<10, 3> (two fused iterations, interleaved U, V)")
     roofline = {"bound": "hbm", "achieved": round(kt["pair_gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(kt["pair_gbs"] / PEAK_HBM_GBS, 4),
                 "traffic": round(traffic) if traffic else None,
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                 "traffic_gbs": round(traffic / (kt["pair_us"] * 1e-6) / 1e9, 1) if traffic else None,
                 "traffic_frac": round(traffic / (kt["pair_us"] * 1e-6) / 1e9 / PEAK_HBM_GBS, 4) if traffic else None,
-                "kernel": "energy_pair_kernel<10> (two fused energy + adjoint + Adam iterations, 1000x1000)",
+                "kernel": "energy_pair_kernel<10, 3> (two fused energy + adjoint + Adam iterations, interleaved "
+                          "(U, V) in and out, 1000x1000)",
                 "algorithmic_bytes_per_launch": 2 * BYTES_FUSED_ITERATION * cells,
                 "bytes_per_cell_per_iteration": BYTES_FUSED_ITERATION, "iterations_per_launch": 2,
                 "us_per_launch": round(kt["pair_us"], 2), "launches_per_step": int((rf.its + 1) // 2)}
