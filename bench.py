@@ -69,8 +69,17 @@ ENDS = (10.0, 20.0, 30.0, 60.0)
 # 1000 x 1000 field (~1e-8 .. 1e-6: the energy is a mean over 10^6 cells),
 # where the default turns Adam into plain gradient descent on small ones
 EPSS = (1e-7, 1e-9)
+# warm-up variants tried around the two best grid schedules of each length: a
+# geometric ramp over a fraction of the iterations, from lr0 * r up to lr0 or
+# to WARM_PEAK * lr0 (the warm-up lets a larger step follow)
+RAMPS = (0.05, 0.1, 0.15)
+RAMP_FROM = (1.0 / 6.0, 0.25, 0.4)
+WARM_PEAK = 4.0 / 3.0
 TUNE_SEEDS = (21, 22)     # held-out domes1000 fields the schedules are chosen on
 TUNE_TARGET = 0.009       # every held-out field must reach this error
+# a polished held-out field must also pass the energy-gap check with this
+# margin under its threshold (1 %): J_emu - J_check <= GAP_TARGET |J_check|
+GAP_TARGET = 0.008
 REFINE_TOTALS = (100, 150, 200, 250, 300, 400, 500, 700, 1000, 1500, 2000, 3000)
 SOLVER_TOTALS = (150, 200, 250, 300, 400, 500, 700, 1000, 1500, 2000, 3000, 4500)
 CPU_SAMPLE_S = 15.0  # wall time of the CPU solver-iteration sample
@@ -161,42 +170,111 @@ class Field:
 def run_schedule(op, U0, V0, sch):
     from igm_amd.solver import LegacyAdamState, adam_schedule, solve
     n = sch["its"]
+    n_warm = int(sch.get("ramp", 0.0) * n)
     n_flat = int(sch["flat"] * n)
     return solve(op, U0, V0, n, LegacyAdamState(lr=sch["lr0"], beta1=sch["beta1"], beta2=sch["beta2"],
                                                 epsilon=sch["eps"]),
-                 early_stopping=False, lr_schedule=adam_schedule(sch["lr0"], n_flat, n - n_flat, sch["lr_end"]))
+                 early_stopping=False,
+                 lr_schedule=adam_schedule(sch["lr0"], n_flat, n - n_warm - n_flat, sch["lr_end"], n_warmup=n_warm,
+                                           lr_start=sch.get("lr_start", sch["lr0"])))
 
 
-def schedule_search(fields, start, totals, target):
-    """Shortest schedule of the family {Adam (beta1, beta2) in BETAS and
-    epsilon in EPSS; lr0 in LR0S; a fraction in FLATS at lr0, then a
-    geometric decay to a step in ENDS} that brings EVERY field of ``fields``
-    (from ``start(field)``) to an
-    error <= target: the first total of ``totals`` whose best schedule does
-    (best = lowest worst-case error), then bisected to 2 iterations against
-    the total before it.  Returns the schedule dict with the worst-case
-    error ``err`` and the per-field errors ``errs``."""
+def warmup_variants(sch):
+    """The warm-up neighbourhood of a grid schedule: a geometric ramp over a
+    fraction in RAMPS of the iterations from a step lr0 * r, r in RAMP_FROM,
+    up to lr0 or WARM_PEAK * lr0, then its flat part and its decay."""
+    out = []
+    for peak in (1.0, WARM_PEAK):
+        for ramp in RAMPS:
+            for r in RAMP_FROM:
+                lr0 = sch["lr0"] * peak
+                out.append(dict(sch, lr0=lr0, ramp=ramp, lr_start=lr0 * r))
+    return out
+
+
+def check_gap(fd, U, V, check_its, lr, counters):
+    """The energy-gap check of the fallback on a polished field (U, V) of
+    field ``fd``, as the emulated step runs it: nodes with no ice within one
+    node zeroed, ``check_its`` solver iterations with the solver's step
+    ``lr`` and a legacy-Adam step counter at each of ``counters`` (the check
+    keeps one optimizer, whose counter grows by check_its per call); returns
+    the largest (J_emu - J_check) / |J_check|."""
+    from igm_amd.solver import LegacyAdamState, solve
+    near = getattr(fd, "near_ice", None)
+    if near is None:
+        ice = (fd.f32["thk"] > 0).float()[None, None]
+        near = fd.near_ice = torch.nn.functional.max_pool2d(ice, 3, stride=1, padding=1)[0, 0] > 0
+    U0 = torch.where(near, U, 0.0).contiguous()
+    V0 = torch.where(near, V, 0.0).contiguous()
+    gaps = []
+    for t in counters:
+        opt = LegacyAdamState(lr=lr)
+        opt.iterations = int(t)
+        c = solve(fd.op32, U0, V0, check_its, opt, early_stopping=False).costs
+        j_emu = float(c[0])
+        j_chk = min(float(np.min(c)), j_emu)
+        gaps.append((j_emu - j_chk) / abs(j_chk))
+    return max(gaps)
+
+
+def schedule_search(fields, start, totals, target, gap=None, gap_target=None):
+    """Shortest schedule that brings EVERY field of ``fields`` (from
+    ``start(field)``) to an error <= target.  At each trial length: the grid
+    {Adam (beta1, beta2) in BETAS and epsilon in EPSS; lr0 in LR0S; a
+    fraction in FLATS at lr0, then a geometric decay to a step in ENDS}, then
+    the warm-up variants (``warmup_variants``) of its two best schedules; the
+    best is the lowest worst-case error.  With ``gap`` (a function of the
+    field and the schedule's result), a schedule counts only if it is <=
+    ``gap_target`` on every field.  The length is the first of ``totals``
+    whose best schedule meets the target, then bisected to 2 iterations
+    against the total before it.  Returns the schedule dict with the
+    worst-case error ``err``, the per-field errors ``errs`` and, with
+    ``gap``, the per-field ``gaps``."""
+    none = {"err": float("inf")}
+
+    def evaluate(sch, bound):
+        errs, outs = [], []
+        for fd in fields:
+            U0, V0 = start(fd)
+            r = run_schedule(fd.op32, U0, V0, sch)
+            errs.append(fd.err(r.U, r.V))
+            outs.append((fd, r.U, r.V))
+            if max(errs) >= bound:  # cannot enter the two best
+                return None
+        gaps = [gap(*o) for o in outs] if gap is not None else []
+        if gaps and max(gaps) > gap_target:
+            return None
+        return dict(sch, err=max(errs), errs=errs, gaps=gaps)
 
     def trial(total):
-        best = None
+        top = []  # the two best, lowest worst-case error first
+
+        def consider(sch):
+            bound = top[-1]["err"] if len(top) == 2 else float("inf")
+            r = evaluate(sch, bound)
+            if r is not None:
+                top.append(r)
+                top.sort(key=lambda x: x["err"])
+                del top[2:]
+
         for eps in EPSS:
             for b1, b2 in BETAS:
                 for lr0 in LR0S:
                     for ff in FLATS:
                         for lr_end in ENDS:
-                            sch = {"its": total, "lr0": lr0, "flat": ff, "lr_end": lr_end, "beta1": b1, "beta2": b2,
-                                   "eps": eps}
-                            errs = []
-                            for fd in fields:
-                                U0, V0 = start(fd)
-                                r = run_schedule(fd.op32, U0, V0, sch)
-                                errs.append(fd.err(r.U, r.V))
-                                if best is not None and max(errs) >= best["err"]:
-                                    break
-                            if best is None or max(errs) < best["err"]:
-                                best = dict(sch, err=max(errs), errs=errs)
-        log(f"[bench]   {total} its: worst held-out err {best['err']:.4f} (lr0 {best['lr0']}, flat {best['flat']}, "
-            f"end {best['lr_end']}, betas {best['beta1']}, {best['beta2']}, eps {best['eps']:g})")
+                            consider({"its": total, "lr0": lr0, "flat": ff, "lr_end": lr_end, "beta1": b1,
+                                      "beta2": b2, "eps": eps, "ramp": 0.0, "lr_start": lr0})
+        for base in list(top):
+            for sch in warmup_variants(base):
+                consider(sch)
+        if not top:
+            log(f"[bench]   {total} its: no schedule passes the energy-gap check on every held-out field")
+            return dict(none, its=total)
+        best = top[0]
+        gaps = f", worst gap {max(best['gaps']):.2e}" if best["gaps"] else ""
+        log(f"[bench]   {total} its: worst held-out err {best['err']:.4f}{gaps} (lr0 {best['lr0']:g}, warm-up "
+            f"{best['ramp']:g} from {best['lr_start']:g}, flat {best['flat']}, end {best['lr_end']}, betas "
+            f"{best['beta1']}, {best['beta2']}, eps {best['eps']:g})")
         return best
 
     lo, best = None, None
@@ -205,6 +283,8 @@ def schedule_search(fields, start, totals, target):
         if best["err"] <= target:
             break
         lo = total
+    if "lr0" not in best:
+        raise RuntimeError(f"no schedule of up to {totals[-1]} iterations passes the energy-gap check")
     if best["err"] > target or lo is None:
         return best
     hi = best["its"]
@@ -459,12 +539,19 @@ def main():
     inf_ms = tm.ms / args.steps
     log(f"[bench] emulator inference {inf_ms:.3f} ms, rel err {inf_err:.4f} (held-out seeds: "
         f"{', '.join(f'{f.err(f.Ue, f.Ve):.4f}' for f in tune_f)}); choosing the polish on seeds {TUNE_SEEDS}")
-    pol = schedule_search(tune_f, lambda f: (f.Ue, f.Ve), REFINE_TOTALS, TUNE_TARGET)
+    fb = ice.emulator.fallback
+    # the check's optimizer counter over the timed steps (it starts at zero
+    # and grows by check_its per check): its first and its last value
+    n_chk = int(fb.check_its)
+    counters = (n_chk * args.warmup, n_chk * (args.warmup + args.steps - 1))
+    pol = schedule_search(tune_f, lambda f: (f.Ue, f.Ve), REFINE_TOTALS, TUNE_TARGET,
+                          gap=lambda f, U, V: check_gap(f, U, V, n_chk, float(ice.solver.step_size), counters),
+                          gap_target=GAP_TARGET)
     rf = ice.emulator.refine
     rf.its, rf.lr, rf.flat_frac, rf.lr_end, rf.beta1, rf.beta2, rf.epsilon = (
         int(pol["its"]), float(pol["lr0"]), float(pol["flat"]), float(pol["lr_end"]), float(pol["beta1"]),
         float(pol["beta2"]), float(pol["eps"]))
-    fb = ice.emulator.fallback
+    rf.warmup_frac, rf.warmup_lr = float(pol["ramp"]), float(pol["lr_start"])
 
     def emulator_step():
         EM.update_iceflow_emulated(cfg, state)
@@ -604,9 +691,11 @@ def main():
             f"iteration {cpu_it_s:.2f} s ({k} timed) on {torch.get_num_threads()} threads")
 
     def sched_out(s):
-        return {"its": int(s["its"]), "lr0": s["lr0"], "flat_frac": s["flat"], "lr_end": s["lr_end"],
+        return {"its": int(s["its"]), "lr0": s["lr0"], "warmup_frac": s["ramp"], "warmup_from": s["lr_start"],
+                "flat_frac": s["flat"], "lr_end": s["lr_end"],
                 "adam_betas": [s["beta1"], s["beta2"]], "adam_epsilon": s["eps"],
-                "held_out_errs": [round(e, 5) for e in s["errs"]]}
+                "held_out_errs": [round(e, 5) for e in s["errs"]],
+                **({"held_out_energy_gaps": [round(g, 5) for g in s["gaps"]]} if s.get("gaps") else {})}
 
     out = {
         "metric": METRIC,
@@ -624,8 +713,8 @@ def main():
         "config": {"workload": "domes1000: 1000x1000 grid, dx=50 m, Nz=10, 20 seeded domes on a random-field bed "
                                "(seed 2); emulated iceflow update (bf16 MFMA CNN 16x3x3x32 + 1x1, fp32 first "
                                "layer) polished by the fp32 energy solver with a schedule chosen on held-out "
-                               f"fields (seeds {list(TUNE_SEEDS)}, <= {TUNE_TARGET:.1%} on each), then the "
-                               "energy-gap fallback check",
+                               f"fields (seeds {list(TUNE_SEEDS)}, <= {TUNE_TARGET:.1%} on each, energy gap at "
+                               f"the check <= {GAP_TARGET:.1%}), then the energy-gap fallback check",
                    "weights": os.path.basename(wpath),
                    "polish": sched_out(pol),
                    "fallback_check_its": int(fb.check_its), "fallback_rel_energy_gap": float(fb.rel_energy_gap)},

@@ -109,7 +109,7 @@ ICEFLOW = {
         # then a geometric decay to lr_end (0 its: off); beta1, beta2 and
         # epsilon of its Adam (the Keras defaults unless set)
         "refine": {"its": 0, "lr": 3.0, "flat_frac": 0.3, "lr_end": 1.0e-3, "beta1": 0.9,
-                   "beta2": 0.999, "epsilon": 1.0e-7},
+                   "beta2": 0.999, "epsilon": 1.0e-7, "warmup_frac": 0.0, "warmup_lr": 0.0},
     },
 }
 

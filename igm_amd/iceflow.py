@@ -227,9 +227,11 @@ def finalize(cfg, state):
 
 def refine_emulated(cfg, state):
     """Polish the emulated field: ``refine.its`` plain-solver iterations
-    warm-started from state.U, state.V with a fresh legacy-Adam state, step
-    ``refine.lr`` for the first ``flat_frac`` of them, then a geometric decay
-    to ``lr_end``, with Adam's ``beta1``, ``beta2`` and ``epsilon``; the outputs are
+    warm-started from state.U, state.V with a fresh legacy-Adam state: a
+    geometric warm-up from ``warmup_lr`` to ``refine.lr`` over the first
+    ``warmup_frac`` of them (none by default), step ``refine.lr`` for the next
+    ``flat_frac``, then a geometric decay to ``lr_end``, with Adam's
+    ``beta1``, ``beta2`` and ``epsilon``; the outputs are
     rebuilt from the polished field.  A network whose error is a few percent
     reaches the solver's accuracy in a small fraction of the iterations a
     solve from zero needs."""
@@ -241,11 +243,14 @@ def refine_emulated(cfg, state):
         return
     op = energy_operator(cfg, state)
     lr = float(rf["lr"])
+    n_warm = int(float(rf["warmup_frac"]) * n)
     n_flat = int(float(rf["flat_frac"]) * n)
+    lr_start = float(rf["warmup_lr"]) if float(rf["warmup_lr"]) > 0 else lr
     opt = slv.LegacyAdamState(lr=lr, beta1=float(rf["beta1"]), beta2=float(rf["beta2"]),
                               epsilon=float(rf["epsilon"]))
     res = slv.solve(op, state.U, state.V, n, opt, early_stopping=False,
-                    lr_schedule=slv.adam_schedule(lr, n_flat, n - n_flat, float(rf["lr_end"])))
+                    lr_schedule=slv.adam_schedule(lr, n_flat, n - n_warm - n_flat, float(rf["lr_end"]),
+                                                  n_warmup=n_warm, lr_start=lr_start))
     out = velocity_outputs(res.U, res.V, state.thk, state.vert_weight, int(ice.numerics.Nz))
     _set_outputs(cfg, state, out)
     state.iceflow_unmasked = (res.U, res.V)

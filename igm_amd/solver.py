@@ -257,11 +257,20 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     return SolveResult(*field_after(n), costs, n, bool(st.stopped))
 
 
-def adam_schedule(lr0, n_flat, n_decay, lr_end=1e-3):
-    """Step schedule: lr0 for n_flat iterations, then a geometric decay to
-    lr_end over n_decay iterations (held at lr_end afterwards)."""
+def adam_schedule(lr0, n_flat, n_decay, lr_end=1e-3, n_warmup=0, lr_start=None):
+    """Step schedule: a geometric warm-up from lr_start to lr0 over n_warmup
+    iterations (none by default), lr0 for n_flat iterations, then a geometric
+    decay to lr_end over n_decay iterations (held at lr_end afterwards)."""
     d = (lr_end / lr0) ** (1.0 / max(1, n_decay))
-    return lambda i: lr0 if i < n_flat else max(lr_end, lr0 * d ** (i - n_flat))
+    w = 1.0 if n_warmup <= 0 else (lr0 / (lr0 if lr_start is None else lr_start)) ** (1.0 / n_warmup)
+    ls = lr0 if lr_start is None else lr_start
+
+    def lr(i):
+        if i < n_warmup:
+            return ls * w ** i
+        i -= n_warmup
+        return lr0 if i < n_flat else max(lr_end, lr0 * d ** (i - n_flat))
+    return lr
 
 
 def rel_err(U, V, Ur, Vr, mask):

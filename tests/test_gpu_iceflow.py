@@ -144,6 +144,15 @@ def test_refine_polishes_with_its_schedule_and_adam_betas(dev):
                       lr_schedule=slv.adam_schedule(3.0, 3, 27, 0.3))
     assert not torch.equal(r_def.U, r.U)
     assert state.COST_REFINED < float(op.energy(U0, V0)[0, 3])
+    # with a warm-up: 6 iterations from step 1 up to 3, 3 at 3, 21 decaying
+    state.U, state.V = U0.clone(), V0.clone()
+    rf.warmup_frac, rf.warmup_lr = 0.2, 1.0
+    iceflow.refine_emulated(cfg, state)
+    r_w = slv.solve(op, U0, V0, 30, slv.LegacyAdamState(lr=3.0, beta1=0.95, beta2=0.99), early_stopping=False,
+                    lr_schedule=slv.adam_schedule(3.0, 3, 21, 0.3, n_warmup=6, lr_start=1.0))
+    for got, want in ((state.U, r_w.U), (state.V, r_w.V)):
+        assert torch.equal(got, torch.where(ice_cells, want, torch.zeros_like(want)))
+    assert not torch.equal(r_w.U, r.U)
 
 
 def test_energy_operator_cache_follows_its_inputs(dev):
