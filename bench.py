@@ -31,6 +31,7 @@ Reported next to it:
 * a CPU baseline: the oracle (torch-CPU float32 restatement) on this host.
 """
 import argparse
+import itertools
 import json
 import os
 import sys
@@ -60,27 +61,33 @@ FLOP_PER_PIXEL = 280640.0
 # `bench.py --kernels-only`, scaled by a 1 GiB copy (the roofline's `traffic`)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_domes1000.json")
 # schedule family searched for both paths: Adam (beta1, beta2), initial step,
-# fraction of the iterations at it, step at the end of a geometric decay
-BETAS = ((0.95, 0.99), (0.95, 0.98), (0.95, 0.95), (0.93, 0.95))
-LR0S = (100.0, 120.0, 150.0, 300.0)
+# fraction of the iterations at it, step at the end of a geometric decay, and
+# a geometric warm-up to the initial step over a fraction of the iterations
+# (none, or 10 % from lr0 / 6).  The polish from the emulated field wants small
+# steps (lr0 ~30), the cold start large ones (~150): the family spans both.
+BETAS = ((0.93, 0.95), (0.95, 0.95), (0.95, 0.98))
+LR0S = (20.0, 30.0, 45.0, 70.0, 100.0, 150.0)
 FLATS = (0.1, 0.2, 0.3, 0.5)
-ENDS = (10.0, 20.0, 30.0, 60.0)
-# Adam's epsilon: the Keras default, and one below the per-node gradients of a
-# 1000 x 1000 field (~1e-8 .. 1e-6: the energy is a mean over 10^6 cells),
-# where the default turns Adam into plain gradient descent on small ones
-EPSS = (1e-7, 1e-9)
+ENDS = (2.0, 5.0, 10.0, 20.0, 30.0)
+GRID_RAMPS = ((0.0, 1.0), (0.1, 1.0 / 6.0))
+# Adam's epsilon: below the per-node gradients of a 1000 x 1000 field (~1e-8 ..
+# 1e-6: the energy is a mean over 10^6 cells), where the Keras default 1e-7
+# turns Adam into plain gradient descent on small ones (1e-7 won no length of
+# either leg once both were searched)
+EPSS = (1e-9,)
 # warm-up variants tried around the two best grid schedules of each length: a
 # geometric ramp over a fraction of the iterations, from lr0 * r up to lr0 or
 # to WARM_PEAK * lr0 (the warm-up lets a larger step follow)
 RAMPS = (0.05, 0.1, 0.15)
 RAMP_FROM = (1.0 / 6.0, 0.25, 0.4)
 WARM_PEAK = 4.0 / 3.0
+LOCAL_ROUNDS = 3  # rounds of coordinate moves around the best schedule of a length
 TUNE_SEEDS = (21, 22)     # held-out domes1000 fields the schedules are chosen on
 TUNE_TARGET = 0.009       # every held-out field must reach this error
 # a polished held-out field must also pass the energy-gap check with this
 # margin under its threshold (1 %): J_emu - J_check <= GAP_TARGET |J_check|
 GAP_TARGET = 0.008
-REFINE_TOTALS = (100, 150, 200, 250, 300, 400, 500, 700, 1000, 1500, 2000, 3000)
+REFINE_TOTALS = (80, 100, 120, 150, 200, 250, 300, 400, 500, 700, 1000, 1500, 2000, 3000)
 SOLVER_TOTALS = (150, 200, 250, 300, 400, 500, 700, 1000, 1500, 2000, 3000, 4500)
 CPU_SAMPLE_S = 15.0  # wall time of the CPU solver-iteration sample
 
@@ -217,13 +224,27 @@ def check_gap(fd, U, V, check_its, lr, counters):
     return max(gaps)
 
 
+def local_variants(sch):
+    """Coordinate moves around a schedule: lr0 (with its warm-up start)
+    scaled by 0.8 .. 1.25, the flat fraction +-0.1, the end step scaled by
+    0.5 .. 2 (not above lr0), beta1 and beta2 +-0.02."""
+    out = [dict(sch, lr0=sch["lr0"] * f, lr_start=sch["lr_start"] * f) for f in (0.8, 0.9, 1.1, 1.25)]
+    out += [dict(sch, flat=round(ff, 3)) for ff in (sch["flat"] - 0.1, sch["flat"] + 0.1) if 0.0 <= ff <= 0.8]
+    out += [dict(sch, lr_end=sch["lr_end"] * f) for f in (0.5, 0.7, 1.4, 2.0) if sch["lr_end"] * f <= sch["lr0"]]
+    out += [dict(sch, beta1=round(b, 4)) for b in (sch["beta1"] - 0.02, sch["beta1"] + 0.02) if 0.0 < b < 1.0]
+    out += [dict(sch, beta2=round(b, 4)) for b in (sch["beta2"] - 0.02, sch["beta2"] + 0.02) if 0.0 < b < 1.0]
+    return out
+
+
 def schedule_search(fields, start, totals, target, gap=None, gap_target=None):
     """Shortest schedule that brings EVERY field of ``fields`` (from
     ``start(field)``) to an error <= target.  At each trial length: the grid
-    {Adam (beta1, beta2) in BETAS and epsilon in EPSS; lr0 in LR0S; a
-    fraction in FLATS at lr0, then a geometric decay to a step in ENDS}, then
-    the warm-up variants (``warmup_variants``) of its two best schedules; the
-    best is the lowest worst-case error.  With ``gap`` (a function of the
+    {Adam (beta1, beta2) in BETAS and epsilon in EPSS; a warm-up in
+    GRID_RAMPS; lr0 in LR0S; a fraction in FLATS at lr0, then a geometric
+    decay to a step in ENDS (not above lr0)}, then
+    the warm-up variants (``warmup_variants``) of its two best schedules,
+    then rounds of coordinate moves (``local_variants``) around the best
+    while it improves; the best is the lowest worst-case error.  With ``gap`` (a function of the
     field and the schedule's result), a schedule counts only if it is <=
     ``gap_target`` on every field.  The length is the first of ``totals``
     whose best schedule meets the target, then bisected to 2 iterations
@@ -257,16 +278,21 @@ def schedule_search(fields, start, totals, target, gap=None, gap_target=None):
                 top.sort(key=lambda x: x["err"])
                 del top[2:]
 
-        for eps in EPSS:
-            for b1, b2 in BETAS:
-                for lr0 in LR0S:
-                    for ff in FLATS:
-                        for lr_end in ENDS:
-                            consider({"its": total, "lr0": lr0, "flat": ff, "lr_end": lr_end, "beta1": b1,
-                                      "beta2": b2, "eps": eps, "ramp": 0.0, "lr_start": lr0})
+        for eps, (b1, b2), lr0, ff, lr_end, (ramp, r0) in itertools.product(EPSS, BETAS, LR0S, FLATS, ENDS,
+                                                                            GRID_RAMPS):
+            if lr_end <= lr0:
+                consider({"its": total, "lr0": lr0, "flat": ff, "lr_end": lr_end, "beta1": b1, "beta2": b2,
+                          "eps": eps, "ramp": ramp, "lr_start": lr0 * r0})
         for base in list(top):
             for sch in warmup_variants(base):
                 consider(sch)
+        # then up to LOCAL_ROUNDS rounds of coordinate moves around the best
+        for _ in range(LOCAL_ROUNDS if top else 0):
+            base = top[0]
+            for sch in local_variants(base):
+                consider(sch)
+            if top[0] is base:
+                break
         if not top:
             log(f"[bench]   {total} its: no schedule passes the energy-gap check on every held-out field")
             return dict(none, its=total)

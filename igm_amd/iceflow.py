@@ -321,11 +321,11 @@ def fallback_check(cfg, state):
         return
     res = slv.solve(op, U0, V0, k, state.optimizer, early_stopping=False)
     # the COST of the first iteration is the energy of the field it starts
-    # from: J_emu (float64 0-dim device tensor)
-    j_emu_dev = res.costs_device[0]
-    state.COST_EMULATED = j_emu_dev
-    j_emu = float(j_emu_dev)
-    j_chk = float(min(np.min(res.costs), j_emu))
+    # from: J_emu (float64 0-dim device tensor); one read of the history
+    state.COST_EMULATED = res.costs_device[0]
+    costs = res.costs
+    j_emu = float(costs[0])
+    j_chk = float(min(np.min(costs), j_emu))
     state.COST_CHECK = j_chk
     if j_emu > j_chk + float(fb.rel_energy_gap) * abs(j_chk):
         state.fallback_count += 1

@@ -102,6 +102,31 @@ def upload(chunks, dev):
     return d
 
 
+_TABLES = {}  # step tables of keyed schedules, the most recent _TABLES_MAX
+_TABLES_MAX = 16
+
+
+def step_table(optimizer, t0, n, lr_schedule, dt):
+    """The Adam steps alpha of iterations t0 + 1 .. t0 + n (``optimizer``'s
+    hyper-parameters, step ``lr_schedule(i)`` or the optimizer's lr) in the
+    working precision, contiguous.  Tables of a schedule with a ``key`` (an
+    ``adam_schedule``, or the optimizer's constant lr) are kept: a polish
+    repeated every time step, with a fresh optimizer each time, builds its
+    table once."""
+    sk = getattr(lr_schedule, "key", None) if lr_schedule is not None else ("constant", optimizer.lr)
+    ck = None if sk is None else (sk, t0, n, optimizer.beta1, optimizer.beta2, dt)
+    tab = _TABLES.pop(ck, None) if ck is not None else None
+    if tab is None:
+        lrs = [lr_schedule(i) if lr_schedule is not None else optimizer.lr for i in range(n)]
+        tab = np.ascontiguousarray(optimizer.alphas(t0 + 1, lrs, dt))
+        tab.setflags(write=False)
+        if ck is not None and len(_TABLES) >= _TABLES_MAX:
+            del _TABLES[next(iter(_TABLES))]  # the least recently used
+    if ck is not None:
+        _TABLES[ck] = tab
+    return tab
+
+
 def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early_stopping=True,
           relative_min_delta=0.0002, patience=10, check_every=16, lr_schedule=None, pairs=True):
     """Minimise the energy of ``op`` from (U0, V0) (planar (Nz, Ny, Nx)).
@@ -153,11 +178,10 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     es_size = C.sizeof(L.IgmEarlyStop)
     t0 = optimizer.iterations
     # the Adam step of every iteration, as the legacy optimizer computes it
-    lrs = [lr_schedule(i) if lr_schedule is not None else optimizer.lr for i in range(nbitmax)]
-    alphas = optimizer.alphas(t0 + 1, lrs, dt)
+    alphas = step_table(optimizer, t0, nbitmax, lr_schedule, dt)
     # the two states and the step table in one host-to-device copy (es_size
     # is a multiple of 8, so the table is aligned)
-    up = upload([bytes(es_host) * 2, np.ascontiguousarray(alphas).tobytes()], dev)
+    up = upload([bytes(es_host) * 2, alphas.tobytes()], dev)
     es_dev = up[:2 * es_size]
     E = [es_dev.data_ptr(), es_dev.data_ptr() + es_size]
     P = [op.partials, op.partials_b]
@@ -257,20 +281,26 @@ def solve(op: EnergyOperator, U0, V0, nbitmax, optimizer: LegacyAdamState, early
     return SolveResult(*field_after(n), costs, n, bool(st.stopped))
 
 
-def adam_schedule(lr0, n_flat, n_decay, lr_end=1e-3, n_warmup=0, lr_start=None):
+class adam_schedule:
     """Step schedule: a geometric warm-up from lr_start to lr0 over n_warmup
     iterations (none by default), lr0 for n_flat iterations, then a geometric
-    decay to lr_end over n_decay iterations (held at lr_end afterwards)."""
-    d = (lr_end / lr0) ** (1.0 / max(1, n_decay))
-    w = 1.0 if n_warmup <= 0 else (lr0 / (lr0 if lr_start is None else lr_start)) ** (1.0 / n_warmup)
-    ls = lr0 if lr_start is None else lr_start
+    decay to lr_end over n_decay iterations (held at lr_end afterwards).
+    ``schedule(i)`` is the step of iteration i (0-based); ``key`` names the
+    schedule (equal keys, equal steps), so that ``solve`` can keep the step
+    tables it has built."""
 
-    def lr(i):
-        if i < n_warmup:
-            return ls * w ** i
-        i -= n_warmup
-        return lr0 if i < n_flat else max(lr_end, lr0 * d ** (i - n_flat))
-    return lr
+    def __init__(self, lr0, n_flat, n_decay, lr_end=1e-3, n_warmup=0, lr_start=None):
+        self.lr0, self.n_flat, self.lr_end, self.n_warmup = lr0, n_flat, lr_end, n_warmup
+        self.d = (lr_end / lr0) ** (1.0 / max(1, n_decay))
+        self.w = 1.0 if n_warmup <= 0 else (lr0 / (lr0 if lr_start is None else lr_start)) ** (1.0 / n_warmup)
+        self.ls = lr0 if lr_start is None else lr_start
+        self.key = ("adam_schedule", lr0, n_flat, n_decay, lr_end, n_warmup, self.ls if n_warmup > 0 else None)
+
+    def __call__(self, i):
+        if i < self.n_warmup:
+            return self.ls * self.w ** i
+        i -= self.n_warmup
+        return self.lr0 if i < self.n_flat else max(self.lr_end, self.lr0 * self.d ** (i - self.n_flat))
 
 
 def rel_err(U, V, Ur, Vr, mask):
