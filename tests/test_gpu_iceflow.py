@@ -257,3 +257,50 @@ def test_fallback_keeps_a_good_emulated_field(dev):
     assert state.fallback_count == 0
     gap = (float(state.COST_EMULATED) - state.COST_CHECK) / abs(state.COST_CHECK)
     assert gap <= 0.01
+
+
+# The polish that the 1000 x 1000 benchmark chooses on its held-out fields
+# (domes1000 seeds 21 and 22): 98 iterations, a geometric warm-up over the
+# first 10 % from 3.33 to 20, 40 % at 20, then a geometric decay to 10; Adam
+# (0.93, 0.91), epsilon 1e-9.
+POLISH_DOMES1000 = {"its": 98, "warmup_frac": 0.1, "warmup_lr": 20.0 / 6.0, "lr": 20.0, "flat_frac": 0.4,
+                    "lr_end": 10.0, "beta1": 0.93, "beta2": 0.91, "epsilon": 1e-9}
+
+
+def test_benchmark_polish_generalises_to_an_unseen_field(dev):
+    """The emulated update (this build's trained network, that polish, the
+    energy-gap check) on domes1000 seed 31 -- a field that neither the
+    network's training nor the schedule's choice saw -- is within 1 % of
+    the field's float64 minimiser (28,000 float32 + 2,000 float64
+    iterations) and passes the check."""
+    from igm_amd.energy import EnergyOperator
+    from igm_amd.solver import converged_solution, rel_err
+    print("case domes1000 seed 31: emulated update with the benchmark's polish", flush=True)
+    cfg = config.default_cfg()
+    ice = cfg.processes.iceflow
+    ice.method = "emulated"
+    ice.emulator.retrain_freq = 0
+    for k, v in POLISH_DOMES1000.items():
+        setattr(ice.emulator.refine, k, v)
+    from igm_amd import emulator as EM
+    ice.emulator.pretrained = True
+    ice.emulator.name = EM.os.path.join(EM.EMULATOR_DIR, "igm_amd_cnn_16_32_nz10.safetensors")
+    g = syn.domes1000(seed=31)
+    state = State()
+    for k in ("thk", "usurf", "arrhenius", "slidingco", "dX"):
+        setattr(state, k, g[k])
+    state.it = -1
+    iceflow.initialize(cfg, state)
+    state.it = 0
+    iceflow.update(cfg, state)
+    assert state.fallback_count == 0
+    gap = (float(state.COST_EMULATED) - state.COST_CHECK) / abs(state.COST_CHECK)
+    f32 = {k: torch.as_tensor(g[k], device=dev) for k in ("thk", "usurf", "arrhenius", "slidingco", "dX")}
+    dz = np.asarray(state.dzeta)
+    op32 = EnergyOperator(dz, ice.physics, fields=f32)
+    op64 = EnergyOperator(dz, ice.physics, fields={k: v.double() for k, v in f32.items()}, dtype=torch.float64)
+    z = torch.zeros_like(state.U)
+    Ur, Vr, _, _ = converged_solution(op32, op64, z, z.clone())
+    err = rel_err(state.U, state.V, Ur, Vr, f32["thk"] > 0)
+    print(f"  rel err {err:.4f}, energy gap at the check {gap:.2e}", flush=True)
+    assert err <= 0.01
