@@ -487,7 +487,7 @@ int pass_impl(int mode, int nz, int nbatch, int Ny, int Nx, const T* U, const T*
   a.bs_bytes = (uint32_t)((nbatch > 1 ? bs : 0) * (int64_t)sizeof(T));
   a.plane = (int)plane;
   a.Ny = Ny; a.Nx = Nx;
-  const PassGeom g = pass_geom(nbatch, Ny, Nx, kStripOut);
+  const PassGeom g = one_pass_geom(nbatch, Ny, Nx);
   a.nstrips = g.nstrips; a.nrowblk = g.nrowblk; a.blocks_per_batch = g.blocks;
   a.prev_count = 1;
   a.prev_nblk = g.blocks;
@@ -634,7 +634,7 @@ int igm_energy_eval_f32_noalloc(int32_t mode, int32_t nz, int32_t nbatch, int32_
   IGM_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (energy) or 1 (energy and gradient)");
   IGM_CHECK_ARG(out != nullptr, "null pointer");
   IGM_CHECK_ARG(nbatch >= 1 && nbatch <= 65535 && Ny >= 2 && Nx >= 2, "bad sizes");
-  const igm::PassGeom g = igm::pass_geom(nbatch, Ny, Nx, igm::kStripOut);
+  const igm::PassGeom g = igm::one_pass_geom(nbatch, Ny, Nx);
   const int64_t need = 3 * (int64_t)nbatch * g.blocks;
   if (need > igm::kScratchDoubles) {
     igm::set_error("igm_energy_eval_f32_noalloc: this batch needs " + std::to_string(need) +
@@ -659,8 +659,8 @@ int64_t igm_energy_workspace_doubles(int32_t nbatch, int32_t Ny, int32_t Nx) {
   if (nbatch < 1 || Ny < 2 || Nx < 2) return 0;
   // room for either pass: 3 doubles per block of the one-iteration pass,
   // 6 per block of the two-iteration pass (one field)
-  const igm::PassGeom g = igm::pass_geom(nbatch, Ny, Nx, igm::kStripOut);
-  const igm::PassGeom g2 = igm::pass_geom(1, Ny, Nx, igm::kPairOut);
+  const igm::PassGeom g = igm::one_pass_geom(nbatch, Ny, Nx);
+  const igm::PassGeom g2 = igm::pair_pass_geom(Ny, Nx);
   const int64_t one = 3 * (int64_t)nbatch * g.blocks, two = 6 * (int64_t)g2.blocks;
   return one > two ? one : two;
 }
@@ -717,7 +717,7 @@ int igm_energy_finalize(const double* partials, int32_t nbatch, int32_t Ny, int3
   IGM_CHECK_ARG(partials && out, "null pointer");
   IGM_CHECK_ARG(nbatch >= 1 && Ny >= 2 && Nx >= 2, "bad sizes");
   IGM_CHECK_ARG(es == nullptr || nbatch == 1, "early stopping needs nbatch == 1");
-  const igm::PassGeom g = igm::pass_geom(nbatch, Ny, Nx, igm::kStripOut);
+  const igm::PassGeom g = igm::one_pass_geom(nbatch, Ny, Nx);
   hipLaunchKernelGGL(igm::energy_finalize_kernel, dim3(nbatch), dim3(igm::kFinBlock), 0,
                      (hipStream_t)stream, partials, g.blocks, 1, out, es, cost_hist, iter);
   IGM_CHECK_LAUNCH();
@@ -728,7 +728,7 @@ int igm_energy_finalize_pair(const double* partials, int32_t Ny, int32_t Nx, dou
                              IgmEarlyStop* es, double* cost_hist, int32_t iter, void* stream) {
   IGM_CHECK_ARG(partials && out, "null pointer");
   IGM_CHECK_ARG(Ny >= 2 && Nx >= 2, "bad sizes");
-  const igm::PassGeom g = igm::pass_geom(1, Ny, Nx, igm::kPairOut);
+  const igm::PassGeom g = igm::pair_pass_geom(Ny, Nx);
   hipLaunchKernelGGL(igm::energy_finalize_kernel, dim3(1), dim3(igm::kFinBlock), 0,
                      (hipStream_t)stream, partials, g.blocks, 2, out, es, cost_hist, iter);
   IGM_CHECK_LAUNCH();

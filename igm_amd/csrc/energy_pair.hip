@@ -414,7 +414,7 @@ int pair_impl(int nz, int Ny, int Nx, const float* U, const float* V, const floa
   a.mom = const_cast<float*>(mom_in);
   a.mom_out = mom_out;
   a.coef = coef; a.hscale = hscale; a.partials = partials; a.stop = stop;
-  const PassGeom g = pass_geom(1, Ny, Nx, kPairOut);
+  const PassGeom g = pair_pass_geom(Ny, Nx);
   if (alpha_tab) {
     IGM_CHECK_ARG(nbitmax >= 1, "nbitmax must be >= 1");
     IGM_CHECK_ARG(es_in && es_out && fin_out && es_in != es_out,
@@ -429,7 +429,7 @@ int pair_impl(int nz, int Ny, int Nx, const float* U, const float* V, const floa
                   "fused finalize needs es_in != es_out and out");
     IGM_CHECK_ARG(prev_partials != partials, "prev_partials must not alias partials");
     a.prev_count = prev_count;
-    a.prev_nblk = prev_count == 1 ? pass_geom(1, Ny, Nx, kStripOut).blocks : g.blocks;
+    a.prev_nblk = prev_count == 1 ? one_pass_geom(1, Ny, Nx).blocks : g.blocks;
   }
   a.prev_partials = prev_partials; a.es_in = es_in; a.es_out = es_out; a.fin_out = fin_out;
   a.cost_hist = cost_hist; a.prev_iter = prev_iter;

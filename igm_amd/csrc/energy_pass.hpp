@@ -403,20 +403,32 @@ __device__ __forceinline__ void adam_step(T (&m)[4], T g_u, T g_v, T omb1, T omb
 
 // ------------------------------------------------------------ row split --
 //
-// Waves per launch the row split aims at: 4 per CU of MI355X's 256, one per
-// SIMD, all resident at once, so that every SIMD gets one equally long wave
-// (measured at 1000 x 1000: 1024 waves beat 2048-4096, whose extra halo rows
-// cost more than the latency a second wave would hide).  IGM_ENERGY_WAVES
-// overrides it (for tuning sweeps).
-constexpr int kTargetWaves = 1024;
+// Waves per launch the row split aims at: one full round of MI355X's 256
+// CUs.  The two-iteration pass holds 256 registers per lane, one wave per
+// SIMD: 1024 waves (at 1000 x 1000, 119.5 us per launch against 124.0 at
+// 896 and 193.2 at 1088, where a second round starts).  The one-iteration
+// passes get 2048: two waves per SIMD for the float energy and energy +
+// adjoint passes (164 and 179 registers; energy + adjoint 42.3 against
+// 45.4 us at 1024 and 50.5 at 3072; the fused Adam pass 101.4 against
+// 103.8 us).  One geometry serves every one-iteration pass, whatever its
+// dtype and mode, because the finalize and the workspace size are derived
+// from it.  IGM_ENERGY_WAVES and IGM_PAIR_WAVES override them (for tuning
+// sweeps).
+constexpr int kTargetWaves = 2048;
+constexpr int kPairTargetWaves = 1024;
 constexpr int kMinRows = 4;  // fewer rows per wave would be mostly halo
 
+inline int env_waves(const char* name, int dflt) {
+  const char* e = getenv(name);
+  const int r = e ? atoi(e) : dflt;
+  return (r >= 64 && r <= (1 << 20)) ? r : dflt;
+}
 inline int target_waves() {
-  static int n = [] {
-    const char* e = getenv("IGM_ENERGY_WAVES");
-    const int r = e ? atoi(e) : kTargetWaves;
-    return (r >= 64 && r <= (1 << 20)) ? r : kTargetWaves;
-  }();
+  static int n = env_waves("IGM_ENERGY_WAVES", kTargetWaves);
+  return n;
+}
+inline int pair_target_waves() {
+  static int n = env_waves("IGM_PAIR_WAVES", kPairTargetWaves);
   return n;
 }
 
@@ -424,17 +436,22 @@ struct PassGeom {
   int nstrips, nrowblk, blocks;
 };
 // strips of `strip_out` owned columns; the rows of each strip split evenly
-inline PassGeom pass_geom(int nbatch, int Ny, int Nx, int strip_out) {
+inline PassGeom pass_geom(int nbatch, int Ny, int Nx, int strip_out, int target) {
   PassGeom g;
   g.nstrips = (Nx + strip_out - 1) / strip_out;
   const int64_t per = (int64_t)g.nstrips * nbatch;
-  int64_t want = (target_waves() + per / 2) / per;
+  int64_t want = (target + per / 2) / per;
   const int64_t most = Ny / kMinRows > 1 ? Ny / kMinRows : 1;
   g.nrowblk = (int)(want < 1 ? 1 : (want > most ? most : want));
   const int waves = g.nstrips * g.nrowblk;
   g.blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
   return g;
 }
+// the geometry of the one-iteration passes, and of the two-iteration pass
+inline PassGeom one_pass_geom(int nbatch, int Ny, int Nx) {
+  return pass_geom(nbatch, Ny, Nx, kStripOut, target_waves());
+}
+inline PassGeom pair_pass_geom(int Ny, int Nx) { return pass_geom(1, Ny, Nx, kPairOut, pair_target_waves()); }
 
 // Fills the physics and vertical-grid fields of the argument block (host).
 template <typename T>
