@@ -37,6 +37,13 @@ def test_argument_errors_are_reported():
 
 def test_workspace_sizes():
     lib = L.load()
-    assert lib.igm_energy_workspace_doubles(1, 1000, 1000) > 0
+    # 1000 x 1000: the one-iteration passes split 17 strips of 62 columns into
+    # 120 row blocks (2040 waves, 510 blocks of 4, 3 partial sums each); the
+    # two-iteration pass 17 strips of 60 into 60 (1020 waves, 255 blocks, 6
+    # each): 1530 doubles for either
+    assert lib.igm_energy_workspace_doubles(1, 1000, 1000) == 1530
+    # a batch of 1000: one row block per strip, 5 blocks per entry (more than
+    # the two-iteration pass's 1530)
+    assert lib.igm_energy_workspace_doubles(1000, 1000, 1000) == 3 * 1000 * 5
     assert lib.igm_energy_workspace_doubles(0, 10, 10) == 0
     assert lib.igm_wgrad_workspace_floats(9, 10) == 10 * (9 * 1024 + 32)
