@@ -47,3 +47,27 @@ def test_workspace_sizes():
     assert lib.igm_energy_workspace_doubles(1000, 1000, 1000) == 3 * 1000 * 5
     assert lib.igm_energy_workspace_doubles(0, 10, 10) == 0
     assert lib.igm_wgrad_workspace_floats(9, 10) == 10 * (9 * 1024 + 32)
+
+
+def test_wave_targets_follow_the_environment():
+    """IGM_PAIR_WAVES and IGM_ENERGY_WAVES (read once per process) set the
+    row split, and with it the workspace a caller allocates: at 1000 x 1000,
+    2048 waves for the two-iteration pass give 17 x 120 waves in 510 blocks,
+    6 partial sums each; 4096 for the one-iteration passes 17 x 241 waves in
+    1025 blocks, 3 each.  Run in child processes (host code only)."""
+    import os
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r); from igm_amd import _lib as L; "
+            "print(L.load().igm_energy_workspace_doubles(1, 1000, 1000))" % os.path.dirname(os.path.dirname(
+                os.path.abspath(__file__))))
+
+    def run(**env):
+        e = {k: v for k, v in os.environ.items() if k not in ("IGM_PAIR_WAVES", "IGM_ENERGY_WAVES")}
+        e.update(env)
+        return int(subprocess.run([sys.executable, "-c", code], env=e, capture_output=True, text=True,
+                                  check=True, timeout=120).stdout.strip())
+
+    assert run() == 1530
+    assert run(IGM_PAIR_WAVES="2048") == 6 * 510
+    assert run(IGM_ENERGY_WAVES="4096") == 3 * 1025
